@@ -1,0 +1,260 @@
+/*
+ * mumfim_amd: batched explicit fiber-network RVE solver for MI355X (gfx950).
+ *
+ * C ABI of the drop-in boundary.  It replaces the batched RVE analysis that
+ * the FE^2 micro scale calls once per macro Newton iteration:
+ *
+ *   reference interface                                     replaced by
+ *   ------------------------------------------------------  ---------------------------
+ *   BatchedFiberRVEAnalysisExplicit(networks, strategies)   mf_batch_create
+ *     (src/mumfim/microscale/BatchedFiberRVEAnalysisExplicit.h:80-256)
+ *   BatchedRVEAnalysis::run(F, sigma, update_coords)        mf_batch_run / mf_batch_run_device
+ *     (BatchedRVEAnalysis.h:48-50, BatchedFiberRVEAnalysisExplicit.h:257-320)
+ *   BatchedRVEAnalysis::computeMaterialStiffness(C)         mf_batch_tangent
+ *     (BatchedRVEAnalysis.h:55-56, BatchedFiberRVEAnalysisExplicit.h:321-405)
+ *   compute3DOrientationTensor(omega)                       mf_batch_orientation_3d
+ *     (BatchedRVEAnalysis.h:57-62, batched_analysis/BatchedComputeOrientationTensor.h:130-190)
+ *   compute2DOrientationTensor(normal, omega)               mf_batch_orientation_2d
+ *     (BatchedRVEAnalysis.h:63-69, BatchedComputeOrientationTensor.h:56-128,191-208)
+ *   std::exit(EXIT_FAILURE) on size mismatch                 negative return code + mf_last_error
+ *     (BatchedFiberRVEAnalysisExplicit.h:85-90,324-330)
+ *
+ * Array conventions (all host arrays are C row-major, caller owned):
+ *   F      n x 9  doubles, F[r*9 + 3*i + j] = F_ij (incremental w.r.t. the
+ *          current configuration, as the macro scale sends it); n == 1
+ *          broadcasts one F to every RVE (BatchedFiberRVEAnalysisExplicitBase.h:36).
+ *   sigma  n x 6  doubles, Voigt order [xx, yy, zz, yz, xz, xy].
+ *   C      n x 36 doubles, C[r*36 + 6*j + i] = d sigma_j / d eps_i.
+ *   status n int32 per RVE (MF_STATUS_*), may be NULL.
+ *
+ * Threading: a batch handle is not thread safe; every call on a handle is
+ * serialised on the handle's own HIP stream and blocks until results are
+ * written unless the name says _async.
+ */
+#ifndef MUMFIM_AMD_H
+#define MUMFIM_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---------------------------------------------------- */
+#define MF_OK 0
+#define MF_ERR_INVALID (-1)     /* bad argument or size mismatch            */
+#define MF_ERR_HIP (-2)         /* HIP runtime error (incl. no GPU)         */
+#define MF_ERR_NOMEM (-3)       /* host or device allocation failed         */
+#define MF_ERR_STATE (-4)       /* call sequence error (e.g. tangent first) */
+#define MF_ERR_UNSUPPORTED (-5) /* network too large for any kernel variant */
+#define MF_ERR_IO (-6)          /* file could not be read / parsed          */
+
+/* ---- per-RVE status ---------------------------------------------------- */
+#define MF_STATUS_OK 0
+#define MF_STATUS_NAN 1          /* NaN/Inf in residual or stress            */
+#define MF_STATUS_STEP_CAP 2     /* relax mode hit max_steps before tol      */
+#define MF_STATUS_FALLBACK 3     /* surrogate rejected, explicit solve used  */
+
+/* ---- fiber reactions (FiberReactions.h:85-123, FiberNetworkIO_impl.h:123-145) */
+#define MF_REACTION_LINEAR 0
+#define MF_REACTION_NONLINEAR 1
+
+typedef struct mf_reaction {
+  int32_t type;             /* MF_REACTION_*                                  */
+  int32_t pad_;
+  double radius;            /* fiber area = pi * radius^2                     */
+  double E;                 /* Young's modulus                                */
+  double density;           /* rho                                           */
+  double B;                 /* nonlinear: exponential coefficient             */
+  double length_ratio_trns; /* nonlinear: tension transition lambda_t        */
+  double cmp_ratio_trns;    /* nonlinear: compression transition (1.0)       */
+} mf_reaction;
+
+/* One fiber network (one RVE).  Node coordinates live in the RVE cube
+ * [-0.5, 0.5]^3.  `boundary` lists the prescribed nodes in the order the
+ * reference collects them (RVE_impl.h:6-20).  fiber_reaction may be NULL
+ * (every fiber uses reaction 0). */
+typedef struct mf_network {
+  int32_t n_nodes;
+  int32_t n_fibers;
+  int32_t n_boundary;
+  int32_t n_reactions;
+  const double *coords;          /* n_nodes x 3                               */
+  const int32_t *fibers;         /* n_fibers x 2 node indices                 */
+  const int32_t *fiber_reaction; /* n_fibers reaction ids, or NULL            */
+  const int32_t *boundary;       /* n_boundary node indices                   */
+  const mf_reaction *reactions;  /* n_reactions                               */
+  double scale_factor;           /* multiscale stress scale (FiberNetwork.cc:67) */
+} mf_network;
+
+/* ---- solver configuration ---------------------------------------------- */
+/* MF_MODE_RELAX reproduces the reference loop: affine guess on all nodes,
+ *   damped central difference until the internal-force residual (refreshed
+ *   every dt_update_freq steps) drops to `tolerance`
+ *   (BatchedFiberRVEAnalysisExplicitTeamOuterLoop.h:215-372).
+ * MF_MODE_RAMP runs exactly n_steps steps from the committed state with the
+ *   boundary displacement ramped linearly from 0 to (F - I) x_b. */
+#define MF_MODE_RELAX 0
+#define MF_MODE_RAMP 1
+#define MF_PREC_FP64 0   /* reference arithmetic, FP64 everywhere           */
+#define MF_PREC_FP32C 1  /* FP32 force/update, compensated positions, FP64 stress */
+/* MF_REACT_BATCHED_REFERENCE: every fiber uses reaction 0's E, A, rho and the
+ *   linear law, as the reference batched path does
+ *   (BatchedFiberRVEAnalysisExplicit.h:202-218, ...Base.h:155-172).
+ * MF_REACT_PER_FIBER: each fiber uses its own reaction (linear or nonlinear
+ *   toe law, FiberReactions.cc:5-71) and its own rho*A in the nodal mass. */
+#define MF_REACT_BATCHED_REFERENCE 0
+#define MF_REACT_PER_FIBER 1
+#define MF_TANGENT_FORWARD 0     /* 6 solves, (sigma(h) - sigma)/h          */
+#define MF_TANGENT_CENTRAL 1     /* 12 solves, (sigma(+h) - sigma(-h))/(2h) */
+
+typedef struct mf_solver_params {
+  int32_t mode;                 /* MF_MODE_*                                  */
+  int32_t precision;            /* MF_PREC_* of run()                          */
+  int32_t reaction_policy;      /* MF_REACT_*                                  */
+  int32_t tangent_scheme;       /* MF_TANGENT_*                                */
+  int32_t tangent_precision;    /* MF_PREC_* of the perturbed solves           */
+  int32_t n_steps;              /* ramp mode: steps per solve                 */
+  int32_t max_steps;            /* relax mode: step cap (status STEP_CAP)     */
+  int32_t dt_update_freq;       /* steps between dt / residual refresh (100)  */
+  int32_t block_size;           /* threads per solve; 0 = autotuned           */
+  int32_t pad_;
+  double visc_damp_coeff;       /* eta, mass-proportional damping             */
+  double crit_time_scale_factor;/* dt = factor * min(l) / c (0.8)             */
+  double tolerance;             /* relax mode residual tolerance (1e-6)      */
+  double fd_step;               /* tangent strain step h                      */
+} mf_solver_params;
+
+typedef struct mf_batch mf_batch;
+
+/* Defaults reproduce the reference batched solver: relax mode, FP64,
+ * batched-reference reactions, forward tangent with h = 1e-5, eta = 100,
+ * factor 0.8, tol 1e-6, refresh every 100 steps. */
+void mf_default_params(mf_solver_params *p);
+
+const char *mf_last_error(void);
+int mf_version(void);
+
+/* Build the packed device batch (boundary-last reorder, nodal masses,
+ * incidence lists) and upload it.  Mirrors the reference constructor. */
+int mf_batch_create(const mf_network *nets, int32_t n_rve,
+                    const mf_solver_params *params, mf_batch **out);
+void mf_batch_destroy(mf_batch *b);
+int mf_batch_num_rves(const mf_batch *b);
+int mf_batch_set_params(mf_batch *b, const mf_solver_params *params);
+int mf_batch_get_params(const mf_batch *b, mf_solver_params *params);
+
+/* run(F, sigma, update_coords): host buffers. */
+int mf_batch_run(mf_batch *b, const double *F, int32_t nF, double *sigma,
+                 int32_t update_coords, int32_t *status);
+/* computeMaterialStiffness(C) at the state of the last update run. */
+int mf_batch_tangent(mf_batch *b, double *C, int32_t *status);
+/* run + tangent with every solve (1 + 6 or 1 + 12 per RVE) in one launch
+ * (ramp mode) or in the reference's sequence (relax mode). */
+int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
+                              double *sigma, double *C, int32_t *status);
+
+/* Device-pointer variants: F_dev (nF x 9), sigma_dev (n x 6), status_dev (n)
+ * live in HBM; work is queued on the batch stream and not waited for. */
+int mf_batch_run_device_async(mf_batch *b, const double *F_dev, int32_t nF,
+                              double *sigma_dev, int32_t update_coords,
+                              int32_t *status_dev);
+int mf_batch_synchronize(mf_batch *b);
+/* HIP stream (hipStream_t) the batch queues its work on. */
+void *mf_batch_stream(mf_batch *b);
+/* Kernel time (ms, HIP events on the batch stream) of the explicit-solve
+ * launches of the last run, and how many launches that was. */
+int mf_batch_last_solve_ms(mf_batch *b, double *ms, int32_t *n_launches);
+
+/* Orientation tensors of the current configuration. omega: n x 9. */
+int mf_batch_orientation_3d(mf_batch *b, double *omega);
+int mf_batch_orientation_2d(mf_batch *b, const double *normal, int32_t n_normal,
+                            double *omega);
+
+/* Persistent state, in each network's input node order:
+ * x (current coordinates) and u (displacement), n_nodes x 3 per RVE,
+ * concatenated over RVEs; volume per RVE. */
+int64_t mf_batch_total_nodes(const mf_batch *b);
+int mf_batch_get_state(mf_batch *b, double *x, double *u, double *volume);
+int mf_batch_set_state(mf_batch *b, const double *x, const double *u,
+                       const double *volume);
+
+/* Per-RVE sizes, 4 ints per RVE: nodes, free nodes, boundary nodes, fibers. */
+int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
+
+/* Kernel plan of the batch: for each RVE the variant chosen by the planner
+ * (threads per solve, node slots per thread, incidence lists in LDS or not),
+ * 4 ints per RVE.  Returns the number of distinct launch groups. */
+int mf_batch_plan(mf_batch *b, int32_t *plan);
+
+/* ---- diagnostics (no reference counterpart) ------------------------------ */
+/* Measured ceilings for the roofline: LDS read bandwidth (ds_read_b128 over
+ * every CU) and HBM copy bandwidth (read + write bytes), in bytes/s. */
+int mf_measure_lds_bandwidth(double *bytes_per_s);
+int mf_measure_hbm_bandwidth(int64_t bytes, double *bytes_per_s);
+
+/* ---- synthetic networks (the seeded generator) ------------------------- */
+typedef struct mf_gen_params {
+  uint64_t seed;
+  int32_t n_nodes;            /* nodes uniform in [-0.5, 0.5]^3              */
+  int32_t k_neighbors;        /* fibers = deduplicated k-NN graph edges      */
+  double boundary_band;       /* boundary: 0.5 - max|X_i| <= band            */
+  double radius, E, density;  /* reaction parameters (both reactions)        */
+  double nonlinear_fraction;  /* share of fibers on the nonlinear reaction   */
+  double B, length_ratio_trns;/* nonlinear reaction parameters               */
+} mf_gen_params;
+
+void mf_default_gen_params(mf_gen_params *g);
+/* Generates one network.  Call with NULL buffers to obtain the counts; the
+ * second call fills coords (n_nodes*3), fibers (n_fibers*2), fiber_reaction
+ * (n_fibers), boundary (n_boundary), reactions (2 entries). */
+int mf_generate_network(const mf_gen_params *g, int32_t *n_nodes,
+                        int32_t *n_fibers, int32_t *n_boundary, double *coords,
+                        int32_t *fibers, int32_t *fiber_reaction,
+                        int32_t *boundary, mf_reaction *reactions);
+/* Generates n_rve networks (seed of RVE i = g->seed + i, node count
+ * n_nodes[i] or g->n_nodes when n_nodes is NULL) and creates a batch. */
+int mf_batch_create_generated(const mf_gen_params *g, int32_t n_rve,
+                              const int32_t *n_nodes,
+                              const mf_solver_params *params, mf_batch **out);
+/* Uniform doubles in [0,1) from the SplitMix64 stream (seed, stream). */
+int mf_uniform(uint64_t seed, uint64_t stream, int64_t n, double *out);
+
+/* ---- reference text formats (FiberNetworkIO.cc:47-106, FiberNetworkIO_impl.h:17-193) */
+/* Reads "<nn> <ne> <np>\n x y z ...\n n0 n1 ..." and the optional
+ * "<path>.params" file.  Boundary nodes are those within 1e-8 of a face of
+ * [-0.5, 0.5]^3 (RVE.h:86-103).  Two-call protocol as mf_generate_network;
+ * reactions holds up to max_reactions entries. */
+int mf_load_network(const char *path, int32_t *n_nodes, int32_t *n_fibers,
+                    int32_t *n_boundary, int32_t *n_reactions,
+                    int32_t max_reactions, double *coords, int32_t *fibers,
+                    int32_t *fiber_reaction, int32_t *boundary,
+                    mf_reaction *reactions);
+
+/* ---- learned surrogate (BF16 MFMA MLP 10 -> 256 -> 256 -> 256 -> 6) ------ */
+typedef struct mf_mlp mf_mlp;
+int mf_mlp_create(uint64_t seed, mf_mlp **out);
+void mf_mlp_destroy(mf_mlp *m);
+int mf_mlp_num_params(const mf_mlp *m);
+/* Input/output normalisation: x_n = (x - mean)/std, y = y_n*std + mean. */
+int mf_mlp_set_normalization(mf_mlp *m, const float *in_mean,
+                             const float *in_std, const float *out_mean,
+                             const float *out_std);
+/* Master FP32 weights, packed W1,b1,W2,b2,W3,b3,W4,b4 (W row-major out x in). */
+int mf_mlp_get_weights(mf_mlp *m, float *w);
+int mf_mlp_set_weights(mf_mlp *m, const float *w);
+/* Inference on device buffers: X (n x 10, FP32), Y (n x 6, FP32), optional
+ * dYdX (n x 6 x 10, FP32) from the hand-written input-Jacobian backward. */
+int mf_mlp_forward_device(mf_mlp *m, const float *X_dev, int64_t n,
+                          float *Y_dev, float *J_dev);
+/* One Adam step on a device mini-batch (X n x 10, Y n x 6); returns loss. */
+int mf_mlp_train_step_device(mf_mlp *m, const float *X_dev, const float *Y_dev,
+                             int64_t n, float lr, float *loss);
+/* Gradient of the mean-squared loss on a device mini-batch (no update);
+ * grad has mf_mlp_num_params floats. */
+int mf_mlp_gradient_device(mf_mlp *m, const float *X_dev, const float *Y_dev,
+                           int64_t n, float *grad_host, float *loss);
+void *mf_mlp_stream(mf_mlp *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUMFIM_AMD_H */

@@ -1,0 +1,379 @@
+"""mumfim_amd: batched explicit fiber-network RVE solver on MI355X (gfx950).
+
+Python view of the C ABI in include/mumfim_amd.h (ctypes).  The compute
+path is the HIP library libmumfim_amd.so; this module only marshals arrays.
+`BatchedFiberRVEAnalysisExplicit` mirrors the reference's batched analysis
+(src/mumfim/microscale/BatchedFiberRVEAnalysisExplicit.h:28-417):
+run(F, sigma, update_coords), computeMaterialStiffness(C),
+compute3DOrientationTensor(omega), compute2DOrientationTensor(normal, omega).
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmumfim_amd.so")
+
+MF_OK = 0
+MF_MODE_RELAX, MF_MODE_RAMP = 0, 1
+MF_PREC_FP64, MF_PREC_FP32C = 0, 1
+MF_REACT_BATCHED_REFERENCE, MF_REACT_PER_FIBER = 0, 1
+MF_TANGENT_FORWARD, MF_TANGENT_CENTRAL = 0, 1
+MF_REACTION_LINEAR, MF_REACTION_NONLINEAR = 0, 1
+MF_STATUS_OK, MF_STATUS_NAN, MF_STATUS_STEP_CAP, MF_STATUS_FALLBACK = 0, 1, 2, 3
+
+
+class mf_reaction(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad_", C.c_int32), ("radius", C.c_double),
+                ("E", C.c_double), ("density", C.c_double), ("B", C.c_double),
+                ("length_ratio_trns", C.c_double), ("cmp_ratio_trns", C.c_double)]
+
+
+class mf_network(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_fibers", C.c_int32),
+                ("n_boundary", C.c_int32), ("n_reactions", C.c_int32),
+                ("coords", C.POINTER(C.c_double)), ("fibers", C.POINTER(C.c_int32)),
+                ("fiber_reaction", C.POINTER(C.c_int32)),
+                ("boundary", C.POINTER(C.c_int32)),
+                ("reactions", C.POINTER(mf_reaction)), ("scale_factor", C.c_double)]
+
+
+class mf_solver_params(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("precision", C.c_int32),
+                ("reaction_policy", C.c_int32), ("tangent_scheme", C.c_int32),
+                ("tangent_precision", C.c_int32), ("n_steps", C.c_int32),
+                ("max_steps", C.c_int32), ("dt_update_freq", C.c_int32),
+                ("block_size", C.c_int32), ("pad_", C.c_int32),
+                ("visc_damp_coeff", C.c_double),
+                ("crit_time_scale_factor", C.c_double),
+                ("tolerance", C.c_double), ("fd_step", C.c_double)]
+
+
+class mf_gen_params(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_nodes", C.c_int32),
+                ("k_neighbors", C.c_int32), ("boundary_band", C.c_double),
+                ("radius", C.c_double), ("E", C.c_double), ("density", C.c_double),
+                ("nonlinear_fraction", C.c_double), ("B", C.c_double),
+                ("length_ratio_trns", C.c_double)]
+
+
+# Every exported symbol of include/mumfim_amd.h with its ctypes signature.
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+_FP = C.POINTER(C.c_float)
+SIGNATURES = {
+    "mf_last_error": (C.c_char_p, []),
+    "mf_version": (C.c_int, []),
+    "mf_default_params": (None, [C.POINTER(mf_solver_params)]),
+    "mf_batch_create": (C.c_int, [C.POINTER(mf_network), _I, C.POINTER(mf_solver_params), C.POINTER(_P)]),
+    "mf_batch_destroy": (None, [_P]),
+    "mf_batch_num_rves": (C.c_int, [_P]),
+    "mf_batch_set_params": (C.c_int, [_P, C.POINTER(mf_solver_params)]),
+    "mf_batch_get_params": (C.c_int, [_P, C.POINTER(mf_solver_params)]),
+    "mf_batch_run": (C.c_int, [_P, _D, _I, _D, _I, _IP]),
+    "mf_batch_tangent": (C.c_int, [_P, _D, _IP]),
+    "mf_batch_run_with_tangent": (C.c_int, [_P, _D, _I, _D, _D, _IP]),
+    "mf_batch_run_device_async": (C.c_int, [_P, _P, _I, _P, _I, _P]),
+    "mf_batch_synchronize": (C.c_int, [_P]),
+    "mf_batch_stream": (C.c_void_p, [_P]),
+    "mf_batch_last_solve_ms": (C.c_int, [_P, _D, _IP]),
+    "mf_batch_orientation_3d": (C.c_int, [_P, _D]),
+    "mf_batch_orientation_2d": (C.c_int, [_P, _D, _I, _D]),
+    "mf_batch_total_nodes": (C.c_int64, [_P]),
+    "mf_batch_get_state": (C.c_int, [_P, _D, _D, _D]),
+    "mf_batch_set_state": (C.c_int, [_P, _D, _D, _D]),
+    "mf_batch_plan": (C.c_int, [_P, _IP]),
+    "mf_batch_rve_sizes": (C.c_int, [_P, _IP]),
+    "mf_default_gen_params": (None, [C.POINTER(mf_gen_params)]),
+    "mf_generate_network": (C.c_int, [C.POINTER(mf_gen_params), _IP, _IP, _IP, _D, _IP, _IP, _IP, C.POINTER(mf_reaction)]),
+    "mf_batch_create_generated": (C.c_int, [C.POINTER(mf_gen_params), _I, _IP, C.POINTER(mf_solver_params), C.POINTER(_P)]),
+    "mf_uniform": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _D]),
+    "mf_measure_lds_bandwidth": (C.c_int, [_D]),
+    "mf_measure_hbm_bandwidth": (C.c_int, [C.c_int64, _D]),
+    "mf_load_network": (C.c_int, [C.c_char_p, _IP, _IP, _IP, _IP, _I, _D, _IP, _IP, _IP, C.POINTER(mf_reaction)]),
+    "mf_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    "mf_mlp_destroy": (None, [_P]),
+    "mf_mlp_num_params": (C.c_int, [_P]),
+    "mf_mlp_set_normalization": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
+    "mf_mlp_get_weights": (C.c_int, [_P, _FP]),
+    "mf_mlp_set_weights": (C.c_int, [_P, _FP]),
+    "mf_mlp_forward_device": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "mf_mlp_train_step_device": (C.c_int, [_P, _P, _P, C.c_int64, C.c_float, _FP]),
+    "mf_mlp_gradient_device": (C.c_int, [_P, _P, _P, C.c_int64, _FP, _FP]),
+    "mf_mlp_stream": (C.c_void_p, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libmumfim_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m mumfim_amd.build` "
+                "(the solver has no CPU fallback)")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class MfError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != MF_OK:
+        raise MfError(f"mumfim_amd error {rc}: {lib().mf_last_error().decode()}")
+    return rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_IP) if a is not None else None
+
+
+@dataclass
+class Reaction:
+    type: int = MF_REACTION_LINEAR
+    radius: float = 0.0
+    E: float = 0.0
+    density: float = 0.0
+    B: float = 0.0
+    length_ratio_trns: float = 0.0
+    cmp_ratio_trns: float = 1.0
+
+    def to_c(self):
+        return mf_reaction(self.type, 0, self.radius, self.E, self.density, self.B,
+                           self.length_ratio_trns, self.cmp_ratio_trns)
+
+
+@dataclass
+class Network:
+    """One fiber network (one RVE) in input node order."""
+    coords: np.ndarray                 # (N, 3) float64
+    fibers: np.ndarray                 # (M, 2) int32
+    boundary: np.ndarray               # (Nb,) int32
+    reactions: List[Reaction]
+    fiber_reaction: Optional[np.ndarray] = None  # (M,) int32
+    scale_factor: float = 1.0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_nodes(self):
+        return int(self.coords.shape[0])
+
+    @property
+    def n_fibers(self):
+        return int(self.fibers.shape[0])
+
+    def to_c(self):
+        self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)
+        self.fibers = np.ascontiguousarray(self.fibers, dtype=np.int32)
+        self.boundary = np.ascontiguousarray(self.boundary, dtype=np.int32)
+        fr = None
+        if self.fiber_reaction is not None:
+            self.fiber_reaction = np.ascontiguousarray(self.fiber_reaction, dtype=np.int32)
+            fr = self.fiber_reaction
+        rx = (mf_reaction * len(self.reactions))(*[r.to_c() for r in self.reactions])
+        self._keep = [rx]
+        return mf_network(self.n_nodes, self.n_fibers, len(self.boundary),
+                          len(self.reactions), _dptr(self.coords), _iptr(self.fibers),
+                          _iptr(fr), _iptr(self.boundary), rx, self.scale_factor)
+
+
+def _reaction_from_c(r):
+    return Reaction(r.type, r.radius, r.E, r.density, r.B, r.length_ratio_trns,
+                    r.cmp_ratio_trns)
+
+
+def default_gen_params(**kw):
+    g = mf_gen_params()
+    lib().mf_default_gen_params(C.byref(g))
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def generate_network(seed=1, n_nodes=500, **kw) -> Network:
+    """Seeded synthetic network: uniform nodes, deduplicated k-NN fibers,
+    boundary band 0.02 (see mf_generate_network)."""
+    g = default_gen_params(seed=seed, n_nodes=n_nodes, **kw)
+    n, m, nb = C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib().mf_generate_network(C.byref(g), C.byref(n), C.byref(m), C.byref(nb),
+                                    None, None, None, None, None))
+    X = np.zeros((n.value, 3))
+    fb = np.zeros((m.value, 2), np.int32)
+    fr = np.zeros(m.value, np.int32)
+    bd = np.zeros(nb.value, np.int32)
+    rx = (mf_reaction * 2)()
+    check(lib().mf_generate_network(C.byref(g), C.byref(n), C.byref(m), C.byref(nb),
+                                    _dptr(X), _iptr(fb), _iptr(fr), _iptr(bd), rx))
+    return Network(X, fb, bd, [_reaction_from_c(r) for r in rx], fr)
+
+
+def load_network(path) -> Network:
+    """Reads the reference's network text format and its .params file."""
+    n, m, nb, nr = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    p = str(path).encode()
+    check(lib().mf_load_network(p, C.byref(n), C.byref(m), C.byref(nb), C.byref(nr), 0,
+                                None, None, None, None, None))
+    X = np.zeros((n.value, 3))
+    fb = np.zeros((m.value, 2), np.int32)
+    fr = np.zeros(m.value, np.int32)
+    bd = np.zeros(nb.value, np.int32)
+    rx = (mf_reaction * nr.value)()
+    check(lib().mf_load_network(p, C.byref(n), C.byref(m), C.byref(nb), C.byref(nr),
+                                nr.value, _dptr(X), _iptr(fb), _iptr(fr), _iptr(bd), rx))
+    return Network(X, fb, bd, [_reaction_from_c(r) for r in rx], fr)
+
+
+def uniform(seed, stream, n):
+    out = np.zeros(n)
+    check(lib().mf_uniform(seed, stream, n, _dptr(out)))
+    return out
+
+
+def measure_lds_bandwidth():
+    v = C.c_double()
+    check(lib().mf_measure_lds_bandwidth(C.byref(v)))
+    return v.value
+
+
+def measure_hbm_bandwidth(nbytes=1 << 30):
+    v = C.c_double()
+    check(lib().mf_measure_hbm_bandwidth(nbytes, C.byref(v)))
+    return v.value
+
+
+def default_params(**kw):
+    p = mf_solver_params()
+    lib().mf_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class BatchedFiberRVEAnalysisExplicit:
+    """Host mirror of the reference batched analysis over the C ABI."""
+
+    def __init__(self, networks=None, params=None, *, generated=None, n_nodes=None):
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        if generated is not None:
+            g, n_rve = generated
+            nn = None
+            if n_nodes is not None:
+                self._nn = np.ascontiguousarray(n_nodes, dtype=np.int32)
+                nn = _iptr(self._nn)
+            check(lib().mf_batch_create_generated(C.byref(g), n_rve, nn,
+                                                  C.byref(self.params), C.byref(h)))
+        else:
+            nets = (mf_network * len(networks))(*[n.to_c() for n in networks])
+            check(lib().mf_batch_create(nets, len(networks), C.byref(self.params),
+                                        C.byref(h)))
+        self.h = h
+        self.n = lib().mf_batch_num_rves(h)
+
+    def close(self):
+        if self.h:
+            lib().mf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        check(lib().mf_batch_set_params(self.h, C.byref(self.params)))
+
+    def run(self, F, update_coords=True, return_status=False):
+        F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+        sigma = np.zeros((self.n, 6))
+        status = np.zeros(self.n, np.int32)
+        check(lib().mf_batch_run(self.h, _dptr(F), F.shape[0], _dptr(sigma),
+                                 int(bool(update_coords)), _iptr(status)))
+        return (sigma, status) if return_status else sigma
+
+    def computeMaterialStiffness(self, return_status=False):
+        Cm = np.zeros((self.n, 6, 6))
+        status = np.zeros(self.n, np.int32)
+        check(lib().mf_batch_tangent(self.h, _dptr(Cm), _iptr(status)))
+        return (Cm, status) if return_status else Cm
+
+    def run_with_tangent(self, F, return_status=False):
+        F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+        sigma = np.zeros((self.n, 6))
+        Cm = np.zeros((self.n, 6, 6))
+        status = np.zeros(self.n, np.int32)
+        check(lib().mf_batch_run_with_tangent(self.h, _dptr(F), F.shape[0], _dptr(sigma),
+                                              _dptr(Cm), _iptr(status)))
+        return (sigma, Cm, status) if return_status else (sigma, Cm)
+
+    def compute3DOrientationTensor(self):
+        om = np.zeros((self.n, 3, 3))
+        check(lib().mf_batch_orientation_3d(self.h, _dptr(om)))
+        return om
+
+    def compute2DOrientationTensor(self, normal):
+        nv = np.ascontiguousarray(np.asarray(normal, dtype=np.float64).reshape(-1, 3))
+        om = np.zeros((self.n, 3, 3))
+        check(lib().mf_batch_orientation_2d(self.h, _dptr(nv), nv.shape[0], _dptr(om)))
+        return om
+
+    def get_state(self):
+        nn = lib().mf_batch_total_nodes(self.h)
+        x = np.zeros((nn, 3))
+        u = np.zeros((nn, 3))
+        V = np.zeros(self.n)
+        check(lib().mf_batch_get_state(self.h, _dptr(x), _dptr(u), _dptr(V)))
+        return x, u, V
+
+    def last_solve_ms(self):
+        ms = C.c_double()
+        nl = C.c_int32()
+        check(lib().mf_batch_last_solve_ms(self.h, C.byref(ms), C.byref(nl)))
+        return ms.value, nl.value
+
+    def stats(self):
+        """Per-RVE node/fiber counts as numpy arrays N, Nf, Nb, M."""
+        z = np.zeros((self.n, 4), np.int32)
+        check(lib().mf_batch_rve_sizes(self.h, _iptr(z)))
+        return {"N": z[:, 0].astype(np.int64), "Nf": z[:, 1].astype(np.int64),
+                "Nb": z[:, 2].astype(np.int64), "M": z[:, 3].astype(np.int64)}
+
+    def run_device(self, F_dev, sigma_dev, update_coords=False, status_dev=None):
+        """Queues run() on device buffers (torch tensors or raw pointers);
+        F_dev holds 1 or n_rve row-major 3x3 matrices in float64."""
+        fp = F_dev.data_ptr() if hasattr(F_dev, "data_ptr") else int(F_dev)
+        sp = sigma_dev.data_ptr() if hasattr(sigma_dev, "data_ptr") else int(sigma_dev)
+        nF = (F_dev.numel() // 9) if hasattr(F_dev, "numel") else self.n
+        st = None
+        if status_dev is not None:
+            st = status_dev.data_ptr() if hasattr(status_dev, "data_ptr") else int(status_dev)
+        check(lib().mf_batch_run_device_async(self.h, fp, nF, sp, int(bool(update_coords)), st))
+
+    def synchronize(self):
+        check(lib().mf_batch_synchronize(self.h))
+
+    def plan(self):
+        p = np.zeros((self.n, 4), np.int32)
+        ng = lib().mf_batch_plan(self.h, _iptr(p))
+        return ng, p

@@ -1,0 +1,1231 @@
+// C ABI of the batched RVE analysis: construction (packing + upload), the
+// run/tangent state machine of BatchedFiberRVEAnalysisExplicit
+// (src/mumfim/microscale/BatchedFiberRVEAnalysisExplicit.h:80-416) and the
+// launch planner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mf_internal.hpp"
+#include "mf_kernels.hpp"
+
+namespace mf {
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace mf
+
+using namespace mf;
+
+#define MF_HIP(call)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) {                                            \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));    \
+      return MF_ERR_HIP;                                               \
+    }                                                                  \
+  } while (0)
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * count);
+  }
+  hipError_t upload(const std::vector<T> &v, hipStream_t s) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(),
+                          hipMemcpyHostToDevice, s);
+  }
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    return alloc(count);
+  }
+};
+
+void make_react_dev(const Reaction &q, ReactDev &d) {
+  std::memset(&d, 0, sizeof(d));
+  d.type = q.type;
+  d.E = q.E;
+  d.A = q.area();
+  d.B = q.B;
+  d.lt = q.lt;
+  d.lc = q.lc;
+  const double EA = q.E * d.A;
+  d.EA_half = (float)(0.5 * EA);
+  if (q.type == MF_REACTION_NONLINEAR) {
+    const double alpha = EA / q.B;
+    d.alpha = (float)alpha;
+    d.B_half = (float)(0.5 * q.B);
+    auto ext = [&](double lam, float &K, float &b) {
+      // f(lam) beyond a transition: m*(lam - lt)*l0 + b with m*l0 and b
+      // independent of l0 (FiberReactions.cc:16-62)
+      double g = 0.5 * (lam * lam - 1);
+      K = (float)(alpha * ((1 + lam * lam * q.B) * std::exp(q.B * g) - 1));
+      b = (float)(alpha * lam * (std::exp(q.B * g) - 1));
+    };
+    ext(q.lt, d.Kt, d.bt);
+    ext(q.lc, d.Kc, d.bc);
+    d.lt_f = (float)q.lt;
+    d.lt2 = (float)(q.lt * q.lt);
+    d.lc_f = (float)q.lc;
+    d.lc2 = (float)(q.lc * q.lc);
+  }
+}
+
+// Almansi-strain perturbation F_i = sqrt((I - 2e_i)^-1) with e_i the unit
+// strain of Voigt component i scaled by h (engineering shear), as in
+// BatchedFiberRVEAnalysisExplicit.h:340-389 and RVEAnalysis.cc:15-64.
+void perturbation(int i, double h, double Fp[9]) {
+  double D1 = std::sqrt(1.0 / (1 - 2 * h));
+  const double l1 = 1;
+  const double l2 = (1 + h) / (1 - h * h);
+  const double l3 = (1 - h) / (1 - h * h);
+  double l2pl3 = 0.5 * (std::sqrt(l2) + std::sqrt(l3));
+  double l2ml3 = 0.5 * (std::sqrt(l2) - std::sqrt(l3));
+  for (int k = 0; k < 9; ++k) Fp[k] = 0;
+  switch (i) {
+    case 0: Fp[0] = D1; Fp[4] = 1; Fp[8] = 1; break;
+    case 1: Fp[0] = 1; Fp[4] = D1; Fp[8] = 1; break;
+    case 2: Fp[0] = 1; Fp[4] = 1; Fp[8] = D1; break;
+    case 3: Fp[0] = l1; Fp[4] = l2pl3; Fp[8] = l2pl3; Fp[5] = l2ml3; Fp[7] = l2ml3; break;
+    case 4: Fp[0] = l2pl3; Fp[4] = l1; Fp[8] = l2pl3; Fp[2] = l2ml3; Fp[6] = l2ml3; break;
+    default: Fp[0] = l2pl3; Fp[4] = l2pl3; Fp[8] = l1; Fp[1] = l2ml3; Fp[3] = l2ml3; break;
+  }
+}
+
+void matmul3(const double A[9], const double B[9], double C[9]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += A[3 * i + k] * B[3 * k + j];
+      C[3 * i + j] = s;
+    }
+}
+
+bool params_valid(const mf_solver_params &p) {
+  return (p.mode == MF_MODE_RELAX || p.mode == MF_MODE_RAMP) &&
+         (p.precision == MF_PREC_FP64 || p.precision == MF_PREC_FP32C) &&
+         (p.tangent_precision == MF_PREC_FP64 ||
+          p.tangent_precision == MF_PREC_FP32C) &&
+         (p.reaction_policy == MF_REACT_BATCHED_REFERENCE ||
+          p.reaction_policy == MF_REACT_PER_FIBER) &&
+         (p.tangent_scheme == MF_TANGENT_FORWARD ||
+          p.tangent_scheme == MF_TANGENT_CENTRAL) &&
+         p.dt_update_freq >= 1 && p.max_steps >= 1 &&
+         (p.mode != MF_MODE_RAMP || p.n_steps >= 1) && p.fd_step > 0 &&
+         p.fd_step < 0.25 && p.crit_time_scale_factor > 0 &&
+         p.visc_damp_coeff >= 0 &&
+         (p.block_size == 0 || p.block_size == 256 || p.block_size == 512 ||
+          p.block_size == 1024);
+}
+
+template <class F>
+void parallel_for(int n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4 || nt == 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> ts;
+  std::atomic<int> next{0};
+  for (unsigned t = 0; t < nt; ++t)
+    ts.emplace_back([&] {
+      for (int i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &t : ts) t.join();
+}
+
+}  // namespace
+
+struct mf_batch {
+  mf_solver_params p{};
+  int R = 0;
+  int policy = 0;
+  std::vector<PackedRVE> host;
+  std::vector<RveDev> rdev;
+  int64_t total_nodes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0;
+  int last_launches = 0;
+  DevBuf<RveDev> rves;
+  DevBuf<double> X, mass, l0, V, cur_stress, F_in;
+  DevBuf<int32_t> deg, slice_base, slice_D, ell_w, fib;
+  DevBuf<double> ell_l0;
+  DevBuf<float> ell_il;
+  DevBuf<ReactDev> reacts;
+  DevBuf<double> xs[2], us[2];
+  int cur = 0;
+  bool have_run = false;   // current_stress_ valid
+  bool have_prev = false;  // ramp mode: state before the last update run kept
+  std::vector<double> lastF;
+  // solve scratch
+  DevBuf<int32_t> solve_rve, solve_flags, order, status, steps;
+  DevBuf<double> solve_F, sigma;
+  DevBuf<double> omega, normal;
+  // plan: per RVE a variant index per precision
+  std::vector<int> plan[2];
+  std::vector<size_t> lds[2];
+  // one-solve-per-RVE tables kept on the device: RVE ids, launch order per
+  // precision, flags per (update) and the broadcast/gathered F buffer
+  DevBuf<int32_t> id_rve, id_order[2], id_flags[2];
+  DevBuf<double> id_F;
+  std::vector<std::pair<int, int>> id_groups[2];  // (variant, count) in order
+  std::vector<size_t> id_group_lds[2];              // LDS bytes per group
+  DevBuf<double> lastF_dev;
+  bool lastF_on_device = false;
+  ~mf_batch() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Picks the kernel variant of every RVE for one precision: the fewest
+// threads that cover the free nodes with at most 4 slots per thread, then
+// incidence lists in LDS and double-buffered positions where they fit.
+int make_plan(mf_batch *b, int prec) {
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  b->plan[prec].assign(b->R, -1);
+  b->lds[prec].assign(b->R, 0);
+  for (int r = 0; r < b->R; ++r) {
+    const PackedRVE &p = b->host[r];
+    int best = -1;
+    long best_key = 0;
+    for (int i = 0; i < nv; ++i) {
+      const Variant &v = vs[i];
+      if (v.prec != prec) continue;
+      if (b->p.block_size && v.block != b->p.block_size) continue;
+      if ((long)v.block * v.npt < p.Nf) continue;
+      size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
+      if (L > kMaxLds) continue;
+      // smaller thread count first, then fewer slots, then richer flags
+      long key = (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
+      if (best < 0 || key < best_key) {
+        best = i;
+        best_key = key;
+      }
+    }
+    if (best < 0) {
+      set_error("RVE " + std::to_string(r) + " (" + std::to_string(p.N) +
+                " nodes) fits no kernel variant");
+      return MF_ERR_UNSUPPORTED;
+    }
+    b->plan[prec][r] = best;
+    b->lds[prec][r] = lds_bytes(vs[best], p.N, p.Nb, p.ell_free_entries);
+  }
+  return MF_OK;
+}
+
+int upload_batch(mf_batch *b) {
+  const int R = b->R;
+  std::vector<RveDev> rd(R);
+  std::vector<double> X, mass, l0s, ell_l0;
+  std::vector<int32_t> deg, sb, sD, ew, fib;
+  std::vector<float> il;
+  std::vector<ReactDev> rx;
+  int64_t node_off = 0, ell_off = 0, fib_off = 0;
+  int slice_off = 0;
+  for (int r = 0; r < R; ++r) {
+    const PackedRVE &p = b->host[r];
+    RveDev &d = rd[r];
+    std::memset(&d, 0, sizeof(d));
+    d.N = p.N;
+    d.Nf = p.Nf;
+    d.Nb = p.Nb;
+    d.M = p.M;
+    d.node_off = node_off;
+    d.ell_off = ell_off;
+    d.fib_off = fib_off;
+    d.slice_off = slice_off;
+    d.n_slices = (int)p.slice_base.size();
+    d.ell_free_entries = p.ell_free_entries;
+    d.react_off = (int)rx.size();
+    d.n_react = (int)p.reactions.size();
+    d.c_sound = p.c_sound;
+    d.scale = p.scale;
+    X.insert(X.end(), p.X.begin(), p.X.end());
+    mass.insert(mass.end(), p.mass.begin(), p.mass.end());
+    deg.insert(deg.end(), p.deg.begin(), p.deg.end());
+    sb.insert(sb.end(), p.slice_base.begin(), p.slice_base.end());
+    sD.insert(sD.end(), p.slice_D.begin(), p.slice_D.end());
+    ew.insert(ew.end(), p.ell_w.begin(), p.ell_w.end());
+    ell_l0.insert(ell_l0.end(), p.ell_l0.begin(), p.ell_l0.end());
+    for (double v : p.ell_l0) il.push_back((float)(1.0 / v));
+    fib.insert(fib.end(), p.fib.begin(), p.fib.end());
+    for (const Reaction &q : p.reactions) {
+      ReactDev rdv;
+      make_react_dev(q, rdv);
+      rx.push_back(rdv);
+    }
+    node_off += p.N;
+    ell_off += (int64_t)p.ell_w.size();
+    fib_off += p.M;
+    slice_off += d.n_slices;
+  }
+  b->rdev = rd;
+  b->total_nodes = node_off;
+  hipStream_t s = b->stream;
+  MF_HIP(b->rves.upload(rd, s));
+  MF_HIP(b->X.upload(X, s));
+  MF_HIP(b->mass.upload(mass, s));
+  MF_HIP(b->deg.upload(deg, s));
+  MF_HIP(b->slice_base.upload(sb, s));
+  MF_HIP(b->slice_D.upload(sD, s));
+  MF_HIP(b->ell_w.upload(ew, s));
+  MF_HIP(b->ell_l0.upload(ell_l0, s));
+  MF_HIP(b->ell_il.upload(il, s));
+  MF_HIP(b->fib.upload(fib, s));
+  MF_HIP(b->reacts.upload(rx, s));
+  // state: x = X, u = 0, V = 1 (constructor, ...Explicit.h:115-118,250-253)
+  for (int k = 0; k < 2; ++k) {
+    MF_HIP(b->xs[k].upload(X, s));
+    MF_HIP(b->us[k].alloc(3 * node_off));
+    MF_HIP(hipMemsetAsync(b->us[k].p, 0, sizeof(double) * 3 * node_off, s));
+  }
+  std::vector<double> ones(R, 1.0);
+  MF_HIP(b->V.upload(ones, s));
+  MF_HIP(b->cur_stress.alloc(6 * (size_t)R));
+  MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+int build_identity_tables(mf_batch *b);
+
+int create_from_networks(std::vector<Network> &nets, const mf_solver_params *params,
+                         mf_batch **out) {
+  if (!out) {
+    set_error("out is NULL");
+    return MF_ERR_INVALID;
+  }
+  *out = nullptr;
+  mf_solver_params p;
+  if (params)
+    p = *params;
+  else
+    mf_default_params(&p);
+  if (!params_valid(p)) {
+    set_error("invalid mf_solver_params");
+    return MF_ERR_INVALID;
+  }
+  std::unique_ptr<mf_batch> b(new mf_batch());
+  b->p = p;
+  b->R = (int)nets.size();
+  b->policy = p.reaction_policy;
+  b->host.resize(b->R);
+  std::vector<int> rc(b->R, MF_OK);
+  std::vector<std::string> errs(b->R);
+  parallel_for(b->R, [&](int r) {
+    rc[r] = pack_rve(nets[r], p.reaction_policy, b->host[r]);
+  });
+  for (int r = 0; r < b->R; ++r)
+    if (rc[r] != MF_OK) {
+      set_error("RVE " + std::to_string(r) + ": invalid network");
+      return rc[r];
+    }
+  for (int prec = 0; prec < 2; ++prec) {
+    int e = make_plan(b.get(), prec);
+    if (e != MF_OK) return e;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device available");
+    return MF_ERR_HIP;
+  }
+  MF_HIP(hipStreamCreate(&b->stream));
+  MF_HIP(hipEventCreate(&b->ev0));
+  MF_HIP(hipEventCreate(&b->ev1));
+  int e = upload_batch(b.get());
+  if (e != MF_OK) return e;
+  e = build_identity_tables(b.get());
+  if (e != MF_OK) return e;
+  *out = b.release();
+  return MF_OK;
+}
+
+// Launches one set of solves.  solve_* arrays live on the host and are
+// uploaded; sigma/status/steps land in the device scratch.
+struct SolveSet {
+  std::vector<int32_t> rve, flags;
+  std::vector<double> F;  // 9 per solve
+  int prec = MF_PREC_FP64;
+};
+
+int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
+               const double *u_in, double *x_out, double *u_out,
+               double *sigma_dev = nullptr, int32_t *status_dev = nullptr) {
+  const int n = (int)S.rve.size();
+  hipStream_t s = b->stream;
+  MF_HIP(b->solve_rve.ensure(n));
+  MF_HIP(b->solve_flags.ensure(n));
+  MF_HIP(b->solve_F.ensure(9 * (size_t)n));
+  MF_HIP(b->order.ensure(n));
+  MF_HIP(b->sigma.ensure(6 * (size_t)n));
+  MF_HIP(b->status.ensure(n));
+  MF_HIP(b->steps.ensure(n));
+  // group by variant, largest RVEs first inside a group
+  const int prec = S.prec;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  std::vector<int32_t> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    int va = b->plan[prec][S.rve[a]], vc = b->plan[prec][S.rve[c]];
+    if (va != vc) return va < vc;
+    return b->host[S.rve[a]].M > b->host[S.rve[c]].M;
+  });
+  MF_HIP(hipMemcpyAsync(b->solve_rve.p, S.rve.data(), sizeof(int32_t) * n,
+                        hipMemcpyHostToDevice, s));
+  MF_HIP(hipMemcpyAsync(b->solve_flags.p, S.flags.data(), sizeof(int32_t) * n,
+                        hipMemcpyHostToDevice, s));
+  MF_HIP(hipMemcpyAsync(b->solve_F.p, S.F.data(), sizeof(double) * 9 * n,
+                        hipMemcpyHostToDevice, s));
+  MF_HIP(hipMemcpyAsync(b->order.p, order.data(), sizeof(int32_t) * n,
+                        hipMemcpyHostToDevice, s));
+  SolveArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.rves = b->rves.p;
+  a.X = b->X.p;
+  a.mass = b->mass.p;
+  a.deg = b->deg.p;
+  a.slice_base = b->slice_base.p;
+  a.slice_D = b->slice_D.p;
+  a.ell_w = b->ell_w.p;
+  a.ell_l0 = b->ell_l0.p;
+  a.ell_il = b->ell_il.p;
+  a.fib = b->fib.p;
+  a.reacts = b->reacts.p;
+  a.x_in = x_in;
+  a.u_in = u_in;
+  a.x_out = x_out;
+  a.u_out = u_out;
+  a.V = b->V.p;
+  a.solve_rve = b->solve_rve.p;
+  a.solve_F = b->solve_F.p;
+  a.solve_flags = b->solve_flags.p;
+  a.sigma = sigma_dev ? sigma_dev : b->sigma.p;
+  a.status = status_dev ? status_dev : b->status.p;
+  a.steps = b->steps.p;
+  a.mode = b->p.mode;
+  a.n_steps = b->p.n_steps;
+  a.max_steps = b->p.max_steps;
+  a.freq = b->p.dt_update_freq;
+  a.eta = b->p.visc_damp_coeff;
+  a.crit = b->p.crit_time_scale_factor;
+  a.tol = b->p.tolerance;
+  MF_HIP(hipEventRecord(b->ev0, s));
+  int launches = 0;
+  for (int i = 0; i < n;) {
+    int v = b->plan[prec][S.rve[order[i]]];
+    int j = i;
+    size_t L = 0;
+    while (j < n && b->plan[prec][S.rve[order[j]]] == v) {
+      L = std::max(L, b->lds[prec][S.rve[order[j]]]);
+      ++j;
+    }
+    a.order = b->order.p + i;
+    hipError_t e = launch_solve(vs[v], a, j - i, L, s);
+    if (e != hipSuccess) {
+      set_error(std::string("solve launch: ") + hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+    ++launches;
+    i = j;
+  }
+  MF_HIP(hipEventRecord(b->ev1, s));
+  b->last_launches = launches;
+  return MF_OK;
+}
+
+void fill_args(mf_batch *b, SolveArgs &a) {
+  std::memset(&a, 0, sizeof(a));
+  a.rves = b->rves.p;
+  a.X = b->X.p;
+  a.mass = b->mass.p;
+  a.deg = b->deg.p;
+  a.slice_base = b->slice_base.p;
+  a.slice_D = b->slice_D.p;
+  a.ell_w = b->ell_w.p;
+  a.ell_l0 = b->ell_l0.p;
+  a.ell_il = b->ell_il.p;
+  a.fib = b->fib.p;
+  a.reacts = b->reacts.p;
+  a.V = b->V.p;
+  a.steps = b->steps.p;
+  a.mode = b->p.mode;
+  a.n_steps = b->p.n_steps;
+  a.max_steps = b->p.max_steps;
+  a.freq = b->p.dt_update_freq;
+  a.eta = b->p.visc_damp_coeff;
+  a.crit = b->p.crit_time_scale_factor;
+  a.tol = b->p.tolerance;
+}
+
+// Device-resident tables for the common case of one solve per RVE.
+int build_identity_tables(mf_batch *b) {
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  std::vector<int32_t> ids(R);
+  for (int r = 0; r < R; ++r) ids[r] = r;
+  MF_HIP(b->id_rve.upload(ids, s));
+  MF_HIP(b->id_F.alloc(9 * (size_t)R));
+  MF_HIP(b->steps.ensure(R));
+  for (int prec = 0; prec < 2; ++prec) {
+    std::vector<int32_t> order(ids);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+      int va = b->plan[prec][a], vc = b->plan[prec][c];
+      if (va != vc) return va < vc;
+      return b->host[a].M > b->host[c].M;
+    });
+    b->id_groups[prec].clear();
+    b->id_group_lds[prec].clear();
+    for (int i = 0; i < R;) {
+      int v = b->plan[prec][order[i]], j = i;
+      size_t L = 0;
+      while (j < R && b->plan[prec][order[j]] == v) {
+        L = std::max(L, b->lds[prec][order[j]]);
+        ++j;
+      }
+      b->id_groups[prec].push_back({v, j - i});
+      b->id_group_lds[prec].push_back(L);
+      i = j;
+    }
+    MF_HIP(b->id_order[prec].upload(order, s));
+    // flags[prec] reused as: [0..R) update=false, [R..2R) update=true
+    std::vector<int32_t> fl(2 * (size_t)R);
+    const bool ramp = b->p.mode == MF_MODE_RAMP;
+    for (int r = 0; r < R; ++r) {
+      fl[r] = ramp ? 0 : kWriteX;
+      fl[R + r] = kWriteX | kWriteU;
+    }
+    MF_HIP(b->id_flags[prec].upload(fl, s));
+  }
+  MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+int finish_timing(mf_batch *b) {
+  MF_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  MF_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  return MF_OK;
+}
+
+int check_F(mf_batch *b, int nF, int update) {
+  if (nF != 1 && nF != b->R) {
+    set_error("F must hold 1 or n_rve deformation gradients");
+    return MF_ERR_INVALID;
+  }
+  // updateVolume requires one F per RVE (...Base.h:136-142)
+  if (update && nF != b->R) {
+    set_error("The volume and deformation gradients must have the same extent "
+              "corresponding to the number of RVEs.");
+    return MF_ERR_INVALID;
+  }
+  return MF_OK;
+}
+
+// run() with F on the device and one solve per RVE: no host round trip.
+int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
+                    double *sigma_dev, int32_t *status_dev) {
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  const bool ramp = b->p.mode == MF_MODE_RAMP;
+  const int prec = b->p.precision;
+  if (update) {
+    hipError_t e = launch_update_volume(F_dev, nF, b->V.p, R, s);
+    if (e != hipSuccess) {
+      set_error(hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+  }
+  const double *Fs = F_dev;
+  if (nF == 1) {
+    hipError_t e = launch_broadcast_F(F_dev, b->id_F.p, R, s);
+    if (e != hipSuccess) {
+      set_error(hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+    Fs = b->id_F.p;
+  }
+  MF_HIP(b->sigma.ensure(6 * (size_t)R));
+  MF_HIP(b->status.ensure(R));
+  SolveArgs a;
+  fill_args(b, a);
+  const int c = b->cur;
+  a.x_in = b->xs[c].p;
+  a.u_in = b->us[c].p;
+  a.x_out = ramp ? b->xs[c ^ 1].p : b->xs[c].p;
+  a.u_out = ramp ? b->us[c ^ 1].p : b->us[c].p;
+  a.solve_rve = b->id_rve.p;
+  a.solve_F = Fs;
+  a.solve_flags = b->id_flags[prec].p + (update ? R : 0);
+  a.sigma = sigma_dev ? sigma_dev : b->sigma.p;
+  a.status = status_dev ? status_dev : b->status.p;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  MF_HIP(hipEventRecord(b->ev0, s));
+  int off = 0, launches = 0;
+  for (auto &g : b->id_groups[prec]) {
+    a.order = b->id_order[prec].p + off;
+    hipError_t e = launch_solve(vs[g.first], a, g.second,
+                                b->id_group_lds[prec][launches], s);
+    if (e != hipSuccess) {
+      set_error(std::string("solve launch: ") + hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+    off += g.second;
+    ++launches;
+  }
+  MF_HIP(hipEventRecord(b->ev1, s));
+  b->last_launches = launches;
+  if (update) {
+    MF_HIP(hipMemcpyAsync(b->cur_stress.p, a.sigma, sizeof(double) * 6 * R,
+                          hipMemcpyDeviceToDevice, s));
+    b->have_run = true;
+    if (ramp) {
+      b->cur ^= 1;
+      b->have_prev = true;
+      MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));
+      MF_HIP(hipMemcpyAsync(b->lastF_dev.p, Fs, sizeof(double) * 9 * R,
+                            hipMemcpyDeviceToDevice, s));
+      b->lastF_on_device = true;
+    }
+  }
+  return MF_OK;
+}
+
+int copy_out(mf_batch *b, double *sigma, int32_t *status, int n) {
+  hipStream_t s = b->stream;
+  if (sigma)
+    MF_HIP(hipMemcpyAsync(sigma, b->sigma.p, sizeof(double) * 6 * n,
+                          hipMemcpyDeviceToHost, s));
+  if (status)
+    MF_HIP(hipMemcpyAsync(status, b->status.p, sizeof(int32_t) * n,
+                          hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+// Tangent: relax mode follows the reference sequence (one trial launch per
+// column, each trial starting from the previous trial's coordinates);
+// ramp mode runs every perturbed solve from the state before the last
+// update run in one launch.  C is left on the device in `Cdev`.
+int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  const double h = b->p.fd_step;
+  const bool central = b->p.tangent_scheme == MF_TANGENT_CENTRAL;
+  if (!b->have_run) {
+    set_error("computeMaterialStiffness needs a previous run with update_coords");
+    return MF_ERR_STATE;
+  }
+  DevBuf<double> sig_p;
+  MF_HIP(sig_p.alloc(6 * (size_t)R));
+  if (b->p.mode == MF_MODE_RELAX) {
+    DevBuf<double> sig_m;
+    if (central) MF_HIP(sig_m.alloc(6 * (size_t)R));
+    std::vector<int32_t> st_all(R, MF_STATUS_OK), st(R);
+    for (int i = 0; i < 6; ++i) {
+      for (int sign = 0; sign < (central ? 2 : 1); ++sign) {
+        double Fp[9];
+        perturbation(i, sign ? -h : h, Fp);
+        SolveSet S;
+        S.prec = b->p.tangent_precision;
+        S.rve.resize(R);
+        S.flags.assign(R, kWriteX);
+        S.F.resize(9 * (size_t)R);
+        for (int r = 0; r < R; ++r) {
+          S.rve[r] = r;
+          std::copy(Fp, Fp + 9, S.F.begin() + 9 * (size_t)r);
+        }
+        const int c = b->cur;
+        int e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c].p, b->us[c].p,
+                           sign ? sig_m.p : sig_p.p);
+        if (e != MF_OK) return e;
+        MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * R,
+                              hipMemcpyDeviceToHost, s));
+        MF_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < R; ++r)
+          if (st_all[r] == MF_STATUS_OK) st_all[r] = st[r];
+      }
+      hipError_t e = central
+          ? launch_fd(sig_p.p, sig_m.p, Cdev, R, i, 2 * h, 6, 6, s)
+          : launch_fd(sig_p.p, b->cur_stress.p, Cdev, R, i, h, 6, 6, s);
+      if (e != hipSuccess) {
+        set_error(hipGetErrorString(e));
+        return MF_ERR_HIP;
+      }
+    }
+    MF_HIP(hipStreamSynchronize(s));
+    if (status_out) *status_out = st_all;
+    return MF_OK;
+  }
+  // ramp mode
+  if (!b->have_prev) {
+    set_error("ramp-mode tangent needs the state of the last update run");
+    return MF_ERR_STATE;
+  }
+  if (b->lastF_on_device) {
+    b->lastF.assign(9 * (size_t)R, 0.0);
+    MF_HIP(hipMemcpyAsync(b->lastF.data(), b->lastF_dev.p, sizeof(double) * 9 * R,
+                          hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    b->lastF_on_device = false;
+  }
+  const int per = central ? 13 : 7;
+  SolveSet S;
+  S.prec = b->p.tangent_precision;
+  for (int r = 0; r < R; ++r) {
+    const double *F0 = b->lastF.data() + 9 * (size_t)r;
+    for (int k = 1; k < per; ++k) {
+      int i = central ? (k - 1) / 2 : k - 1;
+      double sgn = (central && ((k - 1) % 2 == 1)) ? -1.0 : 1.0;
+      double Fp[9], Ft[9];
+      perturbation(i, sgn * h, Fp);
+      matmul3(Fp, F0, Ft);
+      S.rve.push_back(r);
+      S.flags.push_back(0);
+      S.F.insert(S.F.end(), Ft, Ft + 9);
+    }
+  }
+  const int p = b->cur ^ 1;  // state before the last update run
+  int e = launch_set(b, S, b->xs[p].p, b->us[p].p, nullptr, nullptr);
+  if (e != MF_OK) return e;
+  // assemble [base, perturbed...] per RVE for the FD kernel
+  DevBuf<double> all;
+  MF_HIP(all.alloc((size_t)R * per * 6));
+  MF_HIP(hipMemcpy2DAsync(all.p, sizeof(double) * 6 * per, b->cur_stress.p,
+                          sizeof(double) * 6, sizeof(double) * 6, R,
+                          hipMemcpyDeviceToDevice, s));
+  MF_HIP(hipMemcpy2DAsync(all.p + 6, sizeof(double) * 6 * per, b->sigma.p,
+                          sizeof(double) * 6 * (per - 1),
+                          sizeof(double) * 6 * (per - 1), R,
+                          hipMemcpyDeviceToDevice, s));
+  hipError_t he = launch_fd_all(all.p, Cdev, R,
+                                central ? MF_TANGENT_CENTRAL : MF_TANGENT_FORWARD,
+                                h, s);
+  if (he != hipSuccess) {
+    set_error(hipGetErrorString(he));
+    return MF_ERR_HIP;
+  }
+  if (status_out) {
+    std::vector<int32_t> st(S.rve.size());
+    MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * st.size(),
+                          hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    status_out->assign(R, MF_STATUS_OK);
+    for (size_t k = 0; k < st.size(); ++k)
+      if ((*status_out)[S.rve[k]] == MF_STATUS_OK) (*status_out)[S.rve[k]] = st[k];
+  }
+  MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mf_last_error(void) { return g_last_error.c_str(); }
+int mf_version(void) { return 1; }
+
+void mf_default_params(mf_solver_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->mode = MF_MODE_RELAX;
+  p->precision = MF_PREC_FP64;
+  p->reaction_policy = MF_REACT_BATCHED_REFERENCE;
+  p->tangent_scheme = MF_TANGENT_FORWARD;
+  p->tangent_precision = MF_PREC_FP64;
+  p->n_steps = 5000;
+  p->max_steps = 2000000;
+  p->dt_update_freq = 100;
+  p->block_size = 0;
+  p->visc_damp_coeff = 100.0;
+  p->crit_time_scale_factor = 0.8;
+  p->tolerance = 1e-6;
+  p->fd_step = 1e-5;
+}
+
+void mf_default_gen_params(mf_gen_params *g) {
+  std::memset(g, 0, sizeof(*g));
+  g->seed = 1;
+  g->n_nodes = 500;
+  g->k_neighbors = 4;
+  g->boundary_band = 0.02;
+  g->radius = 3.5e-8;
+  g->E = 4.32e7;
+  g->density = 1000.0;
+  g->nonlinear_fraction = 0.0;
+  g->B = 1.2;
+  g->length_ratio_trns = 1.05;
+}
+
+int mf_batch_create(const mf_network *nets, int32_t n_rve,
+                    const mf_solver_params *params, mf_batch **out) {
+  if (!nets || n_rve <= 0) {
+    set_error("mf_batch_create: no networks");
+    return MF_ERR_INVALID;
+  }
+  std::vector<Network> v(n_rve);
+  for (int r = 0; r < n_rve; ++r) {
+    int e = network_from_desc(nets[r], v[r]);
+    if (e != MF_OK) return e;
+  }
+  return create_from_networks(v, params, out);
+}
+
+int mf_batch_create_generated(const mf_gen_params *g, int32_t n_rve,
+                              const int32_t *n_nodes,
+                              const mf_solver_params *params, mf_batch **out) {
+  if (!g || n_rve <= 0) {
+    set_error("mf_batch_create_generated: bad arguments");
+    return MF_ERR_INVALID;
+  }
+  std::vector<Network> v(n_rve);
+  std::vector<int> rc(n_rve, MF_OK);
+  parallel_for(n_rve, [&](int r) {
+    mf_gen_params q = *g;
+    q.seed = g->seed + (uint64_t)r;
+    if (n_nodes) q.n_nodes = n_nodes[r];
+    rc[r] = generate_network(q, v[r]);
+  });
+  for (int r = 0; r < n_rve; ++r)
+    if (rc[r] != MF_OK) return rc[r];
+  return create_from_networks(v, params, out);
+}
+
+void mf_batch_destroy(mf_batch *b) { delete b; }
+int mf_batch_num_rves(const mf_batch *b) { return b ? b->R : 0; }
+
+int mf_batch_set_params(mf_batch *b, const mf_solver_params *params) {
+  if (!b || !params || !params_valid(*params)) {
+    set_error("mf_batch_set_params: invalid");
+    return MF_ERR_INVALID;
+  }
+  if (params->reaction_policy != b->policy) {
+    set_error("reaction_policy is fixed at construction");
+    return MF_ERR_INVALID;
+  }
+  const bool mode_changed = params->mode != b->p.mode;
+  const int old_bs = b->p.block_size;
+  b->p = *params;
+  if (mode_changed) {
+    b->have_prev = false;
+  }
+  if (old_bs != params->block_size)
+    for (int prec = 0; prec < 2; ++prec) {
+      int e = make_plan(b, prec);
+      if (e != MF_OK) return e;
+    }
+  return build_identity_tables(b);
+}
+
+int mf_batch_get_params(const mf_batch *b, mf_solver_params *params) {
+  if (!b || !params) return MF_ERR_INVALID;
+  *params = b->p;
+  return MF_OK;
+}
+
+int mf_batch_run(mf_batch *b, const double *F, int32_t nF, double *sigma,
+                 int32_t update_coords, int32_t *status) {
+  if (!b || !F || !sigma) {
+    set_error("mf_batch_run: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  int e = check_F(b, nF, update_coords);
+  if (e != MF_OK) return e;
+  MF_HIP(b->F_in.ensure(9 * (size_t)nF));
+  MF_HIP(hipMemcpyAsync(b->F_in.p, F, sizeof(double) * 9 * nF,
+                        hipMemcpyHostToDevice, b->stream));
+  e = run_device_fast(b, b->F_in.p, nF, update_coords, nullptr, nullptr);
+  if (e != MF_OK) return e;
+  if (update_coords && b->p.mode == MF_MODE_RAMP) {
+    b->lastF.assign(9 * (size_t)b->R, 0.0);
+    for (int r = 0; r < b->R; ++r)
+      std::copy(F + 9 * (nF == 1 ? 0 : r), F + 9 * (nF == 1 ? 0 : r) + 9,
+                b->lastF.begin() + 9 * (size_t)r);
+    b->lastF_on_device = false;
+  }
+  e = finish_timing(b);
+  if (e != MF_OK) return e;
+  return copy_out(b, sigma, status, b->R);
+}
+
+int mf_batch_run_device_async(mf_batch *b, const double *F_dev, int32_t nF,
+                              double *sigma_dev, int32_t update_coords,
+                              int32_t *status_dev) {
+  if (!b || !F_dev || !sigma_dev) {
+    set_error("mf_batch_run_device_async: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  int e = check_F(b, nF, update_coords);
+  if (e != MF_OK) return e;
+  return run_device_fast(b, F_dev, nF, update_coords, sigma_dev, status_dev);
+}
+
+int mf_batch_synchronize(mf_batch *b) {
+  if (!b) return MF_ERR_INVALID;
+  MF_HIP(hipStreamSynchronize(b->stream));
+  return finish_timing(b);
+}
+
+void *mf_batch_stream(mf_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+int mf_batch_last_solve_ms(mf_batch *b, double *ms, int32_t *n_launches) {
+  if (!b) return MF_ERR_INVALID;
+  if (ms) *ms = b->last_ms;
+  if (n_launches) *n_launches = b->last_launches;
+  return MF_OK;
+}
+
+int mf_batch_tangent(mf_batch *b, double *C, int32_t *status) {
+  if (!b || !C) {
+    set_error("mf_batch_tangent: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  DevBuf<double> Cd;
+  MF_HIP(Cd.alloc(36 * (size_t)b->R));
+  std::vector<int32_t> st;
+  int e = tangent_impl(b, Cd.p, status ? &st : nullptr);
+  if (e != MF_OK) return e;
+  MF_HIP(hipMemcpyAsync(C, Cd.p, sizeof(double) * 36 * b->R,
+                        hipMemcpyDeviceToHost, b->stream));
+  MF_HIP(hipStreamSynchronize(b->stream));
+  if (status) std::copy(st.begin(), st.end(), status);
+  return MF_OK;
+}
+
+int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
+                              double *sigma, double *C, int32_t *status) {
+  if (!b || !F || !sigma || !C) {
+    set_error("mf_batch_run_with_tangent: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  int e = check_F(b, nF, 1);
+  if (e != MF_OK) return e;
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  if (b->p.mode == MF_MODE_RELAX || b->p.precision != b->p.tangent_precision) {
+    e = mf_batch_run(b, F, nF, sigma, 1, status);
+    if (e != MF_OK) return e;
+    std::vector<int32_t> st(R);
+    e = mf_batch_tangent(b, C, status ? st.data() : nullptr);
+    if (e != MF_OK) return e;
+    if (status)
+      for (int r = 0; r < R; ++r)
+        if (status[r] == MF_STATUS_OK) status[r] = st[r];
+    return MF_OK;
+  }
+  // ramp mode: base (committed) + perturbed solves in one launch
+  MF_HIP(b->F_in.ensure(9 * (size_t)nF));
+  MF_HIP(hipMemcpyAsync(b->F_in.p, F, sizeof(double) * 9 * nF,
+                        hipMemcpyHostToDevice, s));
+  hipError_t he = launch_update_volume(b->F_in.p, nF, b->V.p, R, s);
+  if (he != hipSuccess) {
+    set_error(hipGetErrorString(he));
+    return MF_ERR_HIP;
+  }
+  const bool central = b->p.tangent_scheme == MF_TANGENT_CENTRAL;
+  const int per = central ? 13 : 7;
+  const double h = b->p.fd_step;
+  SolveSet S;
+  S.prec = b->p.precision;
+  for (int r = 0; r < R; ++r) {
+    const double *F0 = F + 9 * (nF == 1 ? 0 : r);
+    S.rve.push_back(r);
+    S.flags.push_back(kWriteX | kWriteU);
+    S.F.insert(S.F.end(), F0, F0 + 9);
+    for (int k = 1; k < per; ++k) {
+      int i = central ? (k - 1) / 2 : k - 1;
+      double sgn = (central && ((k - 1) % 2 == 1)) ? -1.0 : 1.0;
+      double Fp[9], Ft[9];
+      perturbation(i, sgn * h, Fp);
+      matmul3(Fp, F0, Ft);
+      S.rve.push_back(r);
+      S.flags.push_back(0);
+      S.F.insert(S.F.end(), Ft, Ft + 9);
+    }
+  }
+  const int c = b->cur;
+  e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c ^ 1].p, b->us[c ^ 1].p);
+  if (e != MF_OK) return e;
+  DevBuf<double> Cd;
+  MF_HIP(Cd.alloc(36 * (size_t)R));
+  he = launch_fd_all(b->sigma.p, Cd.p, R,
+                     central ? MF_TANGENT_CENTRAL : MF_TANGENT_FORWARD, h, s);
+  if (he != hipSuccess) {
+    set_error(hipGetErrorString(he));
+    return MF_ERR_HIP;
+  }
+  MF_HIP(hipMemcpy2DAsync(b->cur_stress.p, sizeof(double) * 6, b->sigma.p,
+                          sizeof(double) * 6 * per, sizeof(double) * 6, R,
+                          hipMemcpyDeviceToDevice, s));
+  MF_HIP(hipMemcpy2DAsync(sigma, sizeof(double) * 6, b->sigma.p,
+                          sizeof(double) * 6 * per, sizeof(double) * 6, R,
+                          hipMemcpyDeviceToHost, s));
+  MF_HIP(hipMemcpyAsync(C, Cd.p, sizeof(double) * 36 * R, hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> st(S.rve.size());
+  MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * st.size(),
+                        hipMemcpyDeviceToHost, s));
+  e = finish_timing(b);
+  if (e != MF_OK) return e;
+  MF_HIP(hipStreamSynchronize(s));
+  b->have_run = true;
+  b->have_prev = true;
+  b->cur ^= 1;
+  b->lastF_on_device = false;
+  b->lastF.assign(9 * (size_t)R, 0.0);
+  for (int r = 0; r < R; ++r)
+    std::copy(F + 9 * (nF == 1 ? 0 : r), F + 9 * (nF == 1 ? 0 : r) + 9,
+              b->lastF.begin() + 9 * (size_t)r);
+  if (status) {
+    for (int r = 0; r < R; ++r) status[r] = MF_STATUS_OK;
+    for (size_t k = 0; k < st.size(); ++k)
+      if (status[S.rve[k]] == MF_STATUS_OK) status[S.rve[k]] = st[k];
+  }
+  return MF_OK;
+}
+
+int mf_batch_orientation_3d(mf_batch *b, double *omega) {
+  return mf_batch_orientation_2d(b, nullptr, 0, omega);
+}
+
+int mf_batch_orientation_2d(mf_batch *b, const double *normal, int32_t n_normal,
+                            double *omega) {
+  if (!b || !omega) return MF_ERR_INVALID;
+  if (normal && n_normal != 1 && n_normal != b->R) {
+    set_error("normal must hold 1 or n_rve vectors");
+    return MF_ERR_INVALID;
+  }
+  hipStream_t s = b->stream;
+  MF_HIP(b->omega.ensure(9 * (size_t)b->R));
+  const double *nd = nullptr;
+  if (normal) {
+    MF_HIP(b->normal.ensure(3 * (size_t)n_normal));
+    MF_HIP(hipMemcpyAsync(b->normal.p, normal, sizeof(double) * 3 * n_normal,
+                          hipMemcpyHostToDevice, s));
+    nd = b->normal.p;
+  }
+  hipError_t e = launch_orientation(b->rves.p, b->xs[b->cur].p, b->fib.p, nd,
+                                    n_normal, b->omega.p, b->R, s);
+  if (e != hipSuccess) {
+    set_error(hipGetErrorString(e));
+    return MF_ERR_HIP;
+  }
+  MF_HIP(hipMemcpyAsync(omega, b->omega.p, sizeof(double) * 9 * b->R,
+                        hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+int64_t mf_batch_total_nodes(const mf_batch *b) { return b ? b->total_nodes : 0; }
+
+int mf_batch_get_state(mf_batch *b, double *x, double *u, double *volume) {
+  if (!b) return MF_ERR_INVALID;
+  const size_t nn = 3 * (size_t)b->total_nodes;
+  std::vector<double> xd(nn), ud(nn);
+  hipStream_t s = b->stream;
+  MF_HIP(hipMemcpyAsync(xd.data(), b->xs[b->cur].p, sizeof(double) * nn,
+                        hipMemcpyDeviceToHost, s));
+  MF_HIP(hipMemcpyAsync(ud.data(), b->us[b->cur].p, sizeof(double) * nn,
+                        hipMemcpyDeviceToHost, s));
+  if (volume)
+    MF_HIP(hipMemcpyAsync(volume, b->V.p, sizeof(double) * b->R,
+                          hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  for (int r = 0; r < b->R; ++r) {
+    const PackedRVE &p = b->host[r];
+    const int64_t off = b->rdev[r].node_off;
+    for (int i = 0; i < p.N; ++i) {
+      const int64_t src = off + p.rank_of_input[i], dst = off + i;
+      for (int d = 0; d < 3; ++d) {
+        if (x) x[3 * dst + d] = xd[3 * src + d];
+        if (u) u[3 * dst + d] = ud[3 * src + d];
+      }
+    }
+  }
+  return MF_OK;
+}
+
+int mf_batch_set_state(mf_batch *b, const double *x, const double *u,
+                       const double *volume) {
+  if (!b) return MF_ERR_INVALID;
+  const size_t nn = 3 * (size_t)b->total_nodes;
+  std::vector<double> xd(nn), ud(nn);
+  for (int r = 0; r < b->R; ++r) {
+    const PackedRVE &p = b->host[r];
+    const int64_t off = b->rdev[r].node_off;
+    for (int i = 0; i < p.N; ++i) {
+      const int64_t dst = off + p.rank_of_input[i], src = off + i;
+      for (int d = 0; d < 3; ++d) {
+        xd[3 * dst + d] = x ? x[3 * src + d] : 0.0;
+        ud[3 * dst + d] = u ? u[3 * src + d] : 0.0;
+      }
+    }
+  }
+  hipStream_t s = b->stream;
+  if (x)
+    MF_HIP(hipMemcpyAsync(b->xs[b->cur].p, xd.data(), sizeof(double) * nn,
+                          hipMemcpyHostToDevice, s));
+  if (u)
+    MF_HIP(hipMemcpyAsync(b->us[b->cur].p, ud.data(), sizeof(double) * nn,
+                          hipMemcpyHostToDevice, s));
+  if (volume)
+    MF_HIP(hipMemcpyAsync(b->V.p, volume, sizeof(double) * b->R,
+                          hipMemcpyHostToDevice, s));
+  MF_HIP(hipStreamSynchronize(s));
+  b->have_prev = false;
+  return MF_OK;
+}
+
+int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes) {
+  if (!b || !sizes) return MF_ERR_INVALID;
+  for (int r = 0; r < b->R; ++r) {
+    const PackedRVE &p = b->host[r];
+    sizes[4 * r + 0] = p.N;
+    sizes[4 * r + 1] = p.Nf;
+    sizes[4 * r + 2] = p.Nb;
+    sizes[4 * r + 3] = p.M;
+  }
+  return MF_OK;
+}
+
+int mf_batch_plan(mf_batch *b, int32_t *plan) {
+  if (!b) return MF_ERR_INVALID;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  const int prec = b->p.precision;
+  std::vector<int> seen;
+  for (int r = 0; r < b->R; ++r) {
+    const Variant &v = vs[b->plan[prec][r]];
+    if (plan) {
+      plan[4 * r + 0] = v.block;
+      plan[4 * r + 1] = v.npt;
+      plan[4 * r + 2] = v.flags;
+      plan[4 * r + 3] = (int32_t)b->lds[prec][r];
+    }
+    if (std::find(seen.begin(), seen.end(), b->plan[prec][r]) == seen.end())
+      seen.push_back(b->plan[prec][r]);
+  }
+  return (int)seen.size();
+}
+
+int mf_generate_network(const mf_gen_params *g, int32_t *n_nodes,
+                        int32_t *n_fibers, int32_t *n_boundary, double *coords,
+                        int32_t *fibers, int32_t *fiber_reaction,
+                        int32_t *boundary, mf_reaction *reactions) {
+  if (!g || !n_nodes || !n_fibers || !n_boundary) {
+    set_error("mf_generate_network: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  Network net;
+  int e = generate_network(*g, net);
+  if (e != MF_OK) return e;
+  *n_nodes = net.N;
+  *n_fibers = net.M;
+  *n_boundary = net.Nb;
+  if (coords) std::copy(net.X.begin(), net.X.end(), coords);
+  if (fibers) std::copy(net.fib.begin(), net.fib.end(), fibers);
+  if (fiber_reaction) std::copy(net.rid.begin(), net.rid.end(), fiber_reaction);
+  if (boundary) std::copy(net.bnd.begin(), net.bnd.end(), boundary);
+  if (reactions)
+    for (size_t i = 0; i < net.reactions.size(); ++i) {
+      const Reaction &q = net.reactions[i];
+      mf_reaction &r = reactions[i];
+      std::memset(&r, 0, sizeof(r));
+      r.type = q.type;
+      r.radius = q.radius;
+      r.E = q.E;
+      r.density = q.rho;
+      r.B = q.B;
+      r.length_ratio_trns = q.lt;
+      r.cmp_ratio_trns = q.lc;
+    }
+  return MF_OK;
+}
+
+int mf_measure_lds_bandwidth(double *bytes_per_s) {
+  if (!bytes_per_s) return MF_ERR_INVALID;
+  MF_HIP(lds_probe(bytes_per_s));
+  return MF_OK;
+}
+
+int mf_measure_hbm_bandwidth(int64_t bytes, double *bytes_per_s) {
+  if (!bytes_per_s || bytes < (1 << 20)) return MF_ERR_INVALID;
+  MF_HIP(hbm_probe(bytes, bytes_per_s));
+  return MF_OK;
+}
+
+int mf_uniform(uint64_t seed, uint64_t stream, int64_t n, double *out) {
+  if (!out || n < 0) return MF_ERR_INVALID;
+  SplitMix64 rng(seed, stream);
+  for (int64_t i = 0; i < n; ++i) out[i] = rng.uniform();
+  return MF_OK;
+}
+
+int mf_load_network(const char *path, int32_t *n_nodes, int32_t *n_fibers,
+                    int32_t *n_boundary, int32_t *n_reactions,
+                    int32_t max_reactions, double *coords, int32_t *fibers,
+                    int32_t *fiber_reaction, int32_t *boundary,
+                    mf_reaction *reactions) {
+  if (!path || !n_nodes || !n_fibers || !n_boundary || !n_reactions) {
+    set_error("mf_load_network: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  Network net;
+  int e = load_network_file(path, net);
+  if (e != MF_OK) return e;
+  *n_nodes = net.N;
+  *n_fibers = net.M;
+  *n_boundary = net.Nb;
+  *n_reactions = (int32_t)net.reactions.size();
+  if (coords) std::copy(net.X.begin(), net.X.end(), coords);
+  if (fibers) std::copy(net.fib.begin(), net.fib.end(), fibers);
+  if (fiber_reaction) std::copy(net.rid.begin(), net.rid.end(), fiber_reaction);
+  if (boundary) std::copy(net.bnd.begin(), net.bnd.end(), boundary);
+  if (reactions) {
+    if ((int)net.reactions.size() > max_reactions) {
+      set_error("reaction buffer too small");
+      return MF_ERR_INVALID;
+    }
+    for (size_t i = 0; i < net.reactions.size(); ++i) {
+      const Reaction &q = net.reactions[i];
+      mf_reaction &r = reactions[i];
+      std::memset(&r, 0, sizeof(r));
+      r.type = q.type;
+      r.radius = q.radius;
+      r.E = q.E;
+      r.density = q.rho;
+      r.B = q.B;
+      r.length_ratio_trns = q.lt;
+      r.cmp_ratio_trns = q.lc;
+    }
+  }
+  return MF_OK;
+}
+
+}  // extern "C"

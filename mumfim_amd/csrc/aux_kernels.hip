@@ -1,0 +1,282 @@
+// Small FP64 kernels around the solve (volume update, finite-difference
+// tangent columns, orientation tensors) and the kernel-variant table.
+// Compiled with -ffp-contract=off.
+#include "kernels_common.hpp"
+
+namespace mf {
+namespace kern {
+
+// ---- small kernels -------------------------------------------------------------
+// LDS read-bandwidth probe: each thread streams float4 reads over a 64 KiB
+// tile in lane-consecutive (conflict-free) order.
+__global__ __launch_bounds__(256) void lds_probe_kernel(float *out, int iters) {
+  __shared__ float4 tile[4096];
+  const int t = threadIdx.x;
+  for (int i = t; i < 4096; i += 256) tile[i] = make_float4(i, 1.0f, 2.0f, 3.0f);
+  __syncthreads();
+  float4 acc = make_float4(0, 0, 0, 0);
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float4 v = tile[(k * 256 + t + it) & 4095];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+  }
+  out[blockIdx.x * 256 + t] = acc.x + acc.y + acc.z + acc.w;
+}
+
+__global__ void copy_probe_kernel(const float4 *in, float4 *out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+__global__ void broadcast_F_kernel(const double *F, double *out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 9 * n) out[i] = F[i % 9];
+}
+
+// V <- det(F) V (updateVolume, ...Base.h:127-154)
+__global__ void update_volume_kernel(const double *F, int nF, double *V,
+                                     int n) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *dg = F + 9 * (nF == 1 ? 0 : i);
+  double J = dg[0] * (dg[4] * dg[8] - dg[5] * dg[7]) -
+             dg[1] * (dg[3] * dg[8] - dg[5] * dg[6]) +
+             dg[2] * (dg[3] * dg[7] - dg[4] * dg[6]);
+  V[i] = J * V[i];
+}
+
+// Finite-difference tangent columns in FP64.  sigma holds, per RVE, the
+// solves [base, col0(+), col1(+), ..., col5(+)] (forward) or
+// [base, col0(+), col0(-), ..., col5(+), col5(-)] (central), 6 values each.
+__global__ void fd_kernel(const double *sigma, double *C, int n, int scheme,
+                          double h) {
+#pragma clang fp contract(off)
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * 36) return;
+  const int r = idx / 36, j = (idx % 36) / 6, i = idx % 6;
+  const int per = scheme == MF_TANGENT_CENTRAL ? 13 : 7;
+  const double *s = sigma + (int64_t)r * per * 6;
+  double v;
+  if (scheme == MF_TANGENT_CENTRAL)
+    v = (s[(1 + 2 * i) * 6 + j] - s[(2 + 2 * i) * 6 + j]) / (2 * h);
+  else
+    v = (s[(1 + i) * 6 + j] - s[j]) / h;
+  C[(int64_t)r * 36 + j * 6 + i] = v;
+}
+
+// One FD column with separate base/perturbed arrays (relax-mode sequence,
+// BatchedFiberRVEAnalysisExplicit.h:394-402).
+__global__ void fd_column_kernel(const double *sig1, const double *sig0,
+                                 double *C, int n, int column, double h,
+                                 int s1, int s0) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int j = 0; j < 6; ++j)
+    C[(int64_t)r * 36 + j * 6 + column] =
+        (sig1[(int64_t)r * s1 + j] - sig0[(int64_t)r * s0 + j]) / h;
+}
+
+// Orientation tensors (BatchedComputeOrientationTensor.h:56-176), one
+// workgroup per RVE, FP64 block reduction.
+template <int B>
+__global__ __launch_bounds__(B) void orientation_kernel(
+    const RveDev *rves, const double *x, const int32_t *fib,
+    const double *normal, int n_normal, double *omega) {
+#pragma clang fp contract(off)
+  __shared__ double red[(B / kWave) * kRedStride * 2];
+  const int rv = blockIdx.x;
+  const RveDev R = rves[rv];
+  double nrm[3] = {0, 0, 0};
+  if (normal) {
+    const double *q = normal + 3 * (n_normal == 1 ? 0 : rv);
+    double n = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    nrm[0] = q[0] / n;
+    nrm[1] = q[1] / n;
+    nrm[2] = q[2] / n;
+  }
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int32_t *fb = fib + 2 * R.fib_off;
+  const double *xs = x + 3 * R.node_off;
+  for (int e = threadIdx.x; e < R.M; e += B) {
+    const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+    double lx = xs[3 * n2] - xs[3 * n1];
+    double ly = xs[3 * n2 + 1] - xs[3 * n1 + 1];
+    double lz = xs[3 * n2 + 2] - xs[3 * n1 + 2];
+    double l = sqrt(lx * lx + ly * ly + lz * lz);
+    lx = lx / l;
+    ly = ly / l;
+    lz = lz / l;
+    if (normal) {
+      double dotn = lx * nrm[0] + ly * nrm[1] + lz * nrm[2];
+      lx = lx - nrm[0] * dotn;
+      ly = ly - nrm[1] * dotn;
+      lz = lz - nrm[2] * dotn;
+      l = sqrt(lx * lx + ly * ly + lz * lz);
+      lx = lx / l;
+      ly = ly / l;
+      lz = lz / l;
+    }
+    const double v[3] = {lx, ly, lz};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) acc[3 * i + j] += v[i] * v[j];
+  }
+  double a6[6] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]};
+  double b3[3] = {acc[6], acc[7], acc[8]};
+  block_sum<B, 6>(a6, red);
+  block_sum<B, 3>(b3, red + (B / kWave) * kRedStride);
+  if (threadIdx.x == 0) {
+    double *o = omega + 9 * rv;
+    for (int i = 0; i < 6; ++i) o[i] = a6[i] / R.M;
+    for (int i = 0; i < 3; ++i) o[6 + i] = b3[i] / R.M;
+  }
+}
+
+
+}  // namespace kern
+
+#define MF_VARIANT_ENTRY(P, B, N, F) {P, B, N, F},
+static const Variant kVariants[] = {MF_VARIANT_LIST(MF_VARIANT_ENTRY)};
+
+const Variant *variants(int *count) {
+  *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+  return kVariants;
+}
+
+size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
+  const size_t Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const size_t NW = v.block / kern::kWave;
+  size_t b = 0;
+  if (v.prec == MF_PREC_FP64) {
+    b += sizeof(double) * 3 * Np * ((v.flags & kDoubleBuf) ? 2 : 1);
+  } else {
+    b += 16 * 2 * Np;
+  }
+  b += sizeof(double) * NW * kern::kRedStride;
+  b += sizeof(double) * 6 * Nbp;
+  b += (sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15);
+  if (v.flags & kEllInLds)
+    b += (size_t)ell_free_entries *
+         (sizeof(int32_t) + (v.prec == MF_PREC_FP64 ? sizeof(double) : sizeof(float)));
+  return b;
+}
+
+hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
+                          size_t lds, hipStream_t s);
+hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
+                          size_t lds, hipStream_t s);
+
+hipError_t launch_solve(const Variant &v, const SolveArgs &a, int n_blocks,
+                        size_t lds, hipStream_t s) {
+  if (n_blocks <= 0) return hipSuccess;
+  return v.prec == MF_PREC_FP64 ? launch_solve64(v, a, n_blocks, lds, s)
+                                : launch_solve32(v, a, n_blocks, lds, s);
+}
+
+hipError_t lds_probe(double *bytes_per_s) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return hipErrorNoDevice;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const int blocks = cus * 8, iters = 4096;
+  float *out = nullptr;
+  e = hipMalloc(&out, sizeof(float) * blocks * 256);
+  if (e != hipSuccess) return e;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  hipLaunchKernelGGL(kern::lds_probe_kernel, dim3(blocks), dim3(256), 0, 0, out, 64);
+  (void)hipEventRecord(a, 0);
+  hipLaunchKernelGGL(kern::lds_probe_kernel, dim3(blocks), dim3(256), 0, 0, out, iters);
+  (void)hipEventRecord(b, 0);
+  e = hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  *bytes_per_s = (double)blocks * 256 * iters * 16 * 16 / (ms * 1e-3);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(out);
+  return e;
+}
+
+hipError_t hbm_probe(int64_t bytes, double *bytes_per_s) {
+  const int64_t n = bytes / 16;
+  float4 *in = nullptr, *out = nullptr;
+  hipError_t e = hipMalloc(&in, n * 16);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(&out, n * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(in);
+    return e;
+  }
+  (void)hipMemset(in, 0, n * 16);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const int blocks = 4096;
+  hipLaunchKernelGGL(kern::copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, in, out, n);
+  (void)hipEventRecord(a, 0);
+  const int reps = 5;
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(kern::copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, in, out, n);
+  (void)hipEventRecord(b, 0);
+  e = hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  *bytes_per_s = 2.0 * n * 16 * reps / (ms * 1e-3);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(in);
+  (void)hipFree(out);
+  return e;
+}
+
+hipError_t launch_broadcast_F(const double *F, double *out, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::broadcast_F_kernel, dim3((9 * n + 255) / 256), dim3(256), 0,
+                     s, F, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_volume(const double *F, int nF, double *V, int n,
+                                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::update_volume_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                     s, F, nF, V, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fd_all(const double *sigma, double *C, int n, int scheme,
+                         double h, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::fd_kernel, dim3((n * 36 + 255) / 256), dim3(256), 0, s,
+                     sigma, C, n, scheme, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_fd(const double *sig_a, const double *sig_b, double *C,
+                     int n, int column, double h, int stride_a, int stride_b,
+                     hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::fd_column_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                     sig_a, sig_b, C, n, column, h, stride_a, stride_b);
+  return hipGetLastError();
+}
+
+hipError_t launch_orientation(const RveDev *rves, const double *x,
+                              const int32_t *fib, const double *normal,
+                              int n_normal, double *omega, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::orientation_kernel<256>, dim3(n), dim3(256), 0, s, rves,
+                     x, fib, normal, n_normal, omega);
+  return hipGetLastError();
+}
+
+}  // namespace mf

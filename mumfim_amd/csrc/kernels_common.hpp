@@ -1,0 +1,162 @@
+// Batched explicit fiber-network RVE solve for gfx950.
+//
+// One workgroup runs one solve (one RVE under one F) for all of its steps.
+// Node positions live in LDS (double-buffered where they fit, so a step needs
+// a single barrier); each thread owns fixed free-node slots whose position,
+// velocity and force stay in registers for the whole solve.  A free node
+// gathers the forces of its incident fibers from a sliced-ELL incidence list
+// (64 degree-sorted nodes per slice = one wavefront, entries in increasing
+// fiber index), which replaces the reference's atomic scatter
+// (BatchedFiberRVEAnalysisExplicitBase.h:226-252) and reproduces its
+// summation order exactly.
+//
+// solve64: FP64 with the reference's operation order and no FMA contraction
+//          (BatchedFiberRVEAnalysisExplicitTeamOuterLoop.h:180-437).
+// solve32c: FP32 stretch/force/update arithmetic on the algebraic form
+//          t = f/l = EA/(2 l0) (lambda^2 - 1), positions carried as
+//          compensated (hi, lo) FP32 pairs updated with TwoSum, stress
+//          reduced in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mf_kernels.hpp"
+
+namespace mf {
+namespace kern {
+
+constexpr int kWave = 64;
+constexpr int kRedStride = 8;  // doubles per wave in the reduction scratch
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, kWave);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+// Sums NV values over the workgroup; every thread receives the totals.  The
+// caller guarantees a barrier separates this call from the previous read of
+// `red`.
+template <int B, int NV>
+__device__ inline void block_sum(double (&v)[NV], double *red) {
+  constexpr int NW = B / kWave;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) red[w * kRedStride + i] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) s += red[k * kRedStride + i];
+    v[i] = s;
+  }
+}
+
+// min of a (slot 0) and sum of b (slot 1) in one pass.
+template <int B>
+__device__ inline void block_min_sum(double &a, double &b, double *red) {
+  constexpr int NW = B / kWave;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double m = wave_min(a), s = wave_sum(b);
+  if (lane == 0) {
+    red[w * kRedStride + 0] = m;
+    red[w * kRedStride + 1] = s;
+  }
+  __syncthreads();
+  m = red[0];
+  s = red[1];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) {
+    double mk = red[k * kRedStride + 0];
+    m = mk < m ? mk : m;
+    s += red[k * kRedStride + 1];
+  }
+  a = m;
+  b = s;
+}
+
+__device__ inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
+
+// ---- reaction laws, FP64 reference formulas (FiberReactions.cc:5-71) ------
+__device__ inline double nl_force(double alpha, double B, double lr,
+                                  double g) {
+  return alpha * lr * (exp(B * g) - 1);
+}
+__device__ inline double nl_deriv(double alpha, double B, double lr, double l0,
+                                  double g) {
+  return alpha / l0 * ((1 + lr * lr * B) * exp(B * g) - 1);
+}
+__device__ inline double react64(const ReactDev &q, double l0, double l) {
+#pragma clang fp contract(off)
+  double lr = l / l0;
+  if (q.type == MF_REACTION_LINEAR) {
+    double g = 1.0 / 2.0 * (lr * lr - 1);
+    return lr * q.E * q.A * g;
+  }
+  double alpha = q.E * q.A / q.B;
+  if (lr > q.lt) {
+    double gt = 1.0 / 2.0 * (q.lt * q.lt - 1);
+    double m = nl_deriv(alpha, q.B, q.lt, l0, gt);
+    double b = nl_force(alpha, q.B, q.lt, gt);
+    return m * (lr - q.lt) * l0 + b;
+  } else if (lr < q.lc) {
+    double gc = 1.0 / 2.0 * (q.lc * q.lc - 1);
+    double m = nl_deriv(alpha, q.B, q.lc, l0, gc);
+    double b = nl_force(alpha, q.B, q.lc, gc);
+    return m * (lr - q.lc) * l0 + b;
+  }
+  double g = 1.0 / 2.0 * (lr * lr - 1);
+  return nl_force(alpha, q.B, lr, g);
+}
+
+// ---- reaction laws, FP32 algebraic forms: t = f/l from lambda^2 ---------
+__device__ inline float react32_t(const ReactDev &q, float lam2, float il) {
+  if (q.type == MF_REACTION_LINEAR) return q.EA_half * il * (lam2 - 1.0f);
+  if (lam2 > q.lt2) {
+    float ilam = __frsqrt_rn(lam2);
+    float lam = lam2 * ilam;
+    return (q.Kt * (lam - q.lt_f) + q.bt) * il * ilam;
+  } else if (lam2 < q.lc2) {
+    float ilam = __frsqrt_rn(lam2);
+    float lam = lam2 * ilam;
+    return (q.Kc * (lam - q.lc_f) + q.bc) * il * ilam;
+  }
+  return q.alpha * il * expm1f(q.B_half * (lam2 - 1.0f));
+}
+
+// Fast path when the whole RVE uses one linear reaction.
+struct Lin64 {
+  double E, A;
+};
+
+}  // namespace kern
+}  // namespace mf
+
+// Compiled kernel variants: (precision, threads, free-node slots per thread,
+// flags kEllInLds|kDoubleBuf).
+#define MF_VARIANT_LIST(X)                                                   \
+  X(MF_PREC_FP64, 256, 1, 3) X(MF_PREC_FP64, 256, 1, 2)                      \
+  X(MF_PREC_FP64, 256, 2, 3) X(MF_PREC_FP64, 256, 2, 2)                      \
+  X(MF_PREC_FP64, 256, 4, 3) X(MF_PREC_FP64, 256, 4, 2)                      \
+  X(MF_PREC_FP64, 256, 4, 0) X(MF_PREC_FP64, 512, 2, 3)                      \
+  X(MF_PREC_FP64, 512, 2, 2) X(MF_PREC_FP64, 512, 4, 2)                      \
+  X(MF_PREC_FP64, 512, 4, 0) X(MF_PREC_FP64, 1024, 2, 2)                     \
+  X(MF_PREC_FP64, 1024, 2, 0) X(MF_PREC_FP64, 1024, 4, 0)                    \
+  X(MF_PREC_FP32C, 256, 1, 3) X(MF_PREC_FP32C, 256, 1, 2)                    \
+  X(MF_PREC_FP32C, 256, 2, 3) X(MF_PREC_FP32C, 256, 2, 2)                    \
+  X(MF_PREC_FP32C, 256, 4, 3) X(MF_PREC_FP32C, 256, 4, 2)                    \
+  X(MF_PREC_FP32C, 512, 2, 3) X(MF_PREC_FP32C, 512, 2, 2)                    \
+  X(MF_PREC_FP32C, 512, 4, 3) X(MF_PREC_FP32C, 512, 4, 2)                    \
+  X(MF_PREC_FP32C, 1024, 2, 2) X(MF_PREC_FP32C, 1024, 4, 2)
+

@@ -1,0 +1,126 @@
+// Internal declarations shared by the host packer, the C ABI and the HIP
+// kernels.  Nothing here is part of the public interface.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mumfim_amd.h"
+
+namespace mf {
+
+// Error reporting for the C ABI (thread-local last error).
+void set_error(const std::string &msg);
+
+// ---- SplitMix64 streams ---------------------------------------------------
+// Each (seed, stream) pair is an independent sequence; doubles carry the top
+// 53 bits so the values are the same on every platform.
+struct SplitMix64 {
+  uint64_t s;
+  static uint64_t mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  SplitMix64(uint64_t seed, uint64_t stream) : s(mix(seed ^ mix(stream))) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
+};
+
+// Stream ids used by the generator.
+enum : uint64_t { kStreamNodes = 1, kStreamReactions = 2 };
+
+struct Reaction {
+  int type = MF_REACTION_LINEAR;
+  double radius = 0, E = 0, rho = 0, B = 0, lt = 0, lc = 1.0;
+  // A = r*r*pi, evaluated as in FiberNetworkIO_impl.h:132.
+  double area() const { return radius * radius * 3.14159265358979323846; }
+};
+
+// Owned copy of one network in input order.
+struct Network {
+  int N = 0, M = 0, Nb = 0;
+  std::vector<double> X;        // 3N
+  std::vector<int32_t> fib;     // 2M
+  std::vector<int32_t> rid;     // M
+  std::vector<int32_t> bnd;     // Nb
+  std::vector<Reaction> reactions;
+  double scale = 1.0;
+};
+
+int network_from_desc(const mf_network &d, Network &out);
+int generate_network(const mf_gen_params &g, Network &out);
+int load_network_file(const std::string &path, Network &out);
+
+// Largest reaction table and ELL degree the kernels accept.
+constexpr int kMaxReactions = 8;
+constexpr int kSlice = 64;  // nodes per ELL slice = one wavefront
+
+// One RVE packed in kernel order: free nodes sorted by degree (descending,
+// stable), then boundary nodes in boundary-list order.
+struct PackedRVE {
+  int N = 0, Nf = 0, Nb = 0, M = 0;
+  std::vector<int32_t> rank_of_input;  // input node -> kernel rank
+  std::vector<int32_t> input_of_rank;
+  std::vector<double> X;               // 3N, kernel order
+  std::vector<double> mass;            // N
+  std::vector<int32_t> deg;            // N
+  // Sliced ELL incidence lists: slice k holds ranks [64k, 64k+64); entry j of
+  // rank r sits at slice_base[k] + 64*j + (r & 63).  Entries of one node are
+  // its incident fibers in increasing fiber index (the reference scatter
+  // order).  Packed word = other-end rank | reaction id << 24.
+  std::vector<int32_t> slice_base, slice_D;
+  std::vector<int32_t> ell_w;
+  std::vector<double> ell_l0;
+  int ell_free_entries = 0;            // entries of slices that hold free nodes
+  std::vector<int32_t> fib;            // 2M kernel ranks, input fiber order
+  std::vector<double> l0;              // M
+  std::vector<int32_t> rid;            // M (after the reaction policy)
+  std::vector<Reaction> reactions;     // after the reaction policy
+  double scale = 1.0;
+  double c_sound = 0.0;                // sqrt(E/rho) used for the time step
+  // Mean fiber length, fiber density, nonlinear fraction (surrogate inputs).
+  double mean_l0 = 0.0, nonlinear_frac = 0.0;
+};
+
+int pack_rve(const Network &net, int reaction_policy, PackedRVE &out);
+
+// ---- device-side layout ------------------------------------------------------
+struct RveDev {
+  int32_t N, Nf, Nb, M;
+  int64_t node_off;   // first node of this RVE in the node arrays
+  int64_t ell_off;    // first ELL entry of this RVE
+  int64_t fib_off;    // first fiber of this RVE
+  int32_t slice_off;  // first slice of this RVE in the slice tables
+  int32_t n_slices;
+  int32_t ell_free_entries;
+  int32_t react_off;
+  int32_t n_react;
+  int32_t pad_;
+  double c_sound, scale;
+};
+
+// Reaction constants as the kernels consume them.
+struct ReactDev {
+  int32_t type;
+  int32_t pad_;
+  // FP64 reference-formula inputs.
+  double E, A, B, lt, lc;
+  // FP32 algebraic forms: t = f/l as a function of lambda^2.
+  float EA_half;   // linear: t = EA/2 * il * (lam2 - 1)
+  float alpha;     // nonlinear alpha = E*A/B
+  float B_half;    // B/2
+  float lt2;       // lambda_t^2
+  float lt_f;
+  float Kt, bt;    // tension extension: f = Kt*(lam - lt) + bt
+  float lc_f, lc2; // compression transition lambda_c, lambda_c^2
+  float Kc, bc;    // compression extension: f = Kc*(lam - lc) + bc
+};
+
+}  // namespace mf

@@ -1,0 +1,80 @@
+// Host-visible interface of the HIP kernels: argument blocks, kernel
+// variants and their LDS footprints.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mf_internal.hpp"
+
+namespace mf {
+
+// Per-solve flags.
+enum : int32_t {
+  kWriteX = 1,  // write final current coordinates to x_out
+  kWriteU = 2,  // write u = x - X to u_out
+};
+
+struct SolveArgs {
+  const RveDev *rves;
+  const double *X;          // 3 per node
+  const double *mass;       // per node
+  const int32_t *deg;       // per node
+  const int32_t *slice_base, *slice_D;
+  const int32_t *ell_w;
+  const double *ell_l0;     // FP64 kernels
+  const float *ell_il;      // FP32 kernels: 1/l0
+  const int32_t *fib;       // 2 per fiber (kernel ranks)
+  const ReactDev *reacts;
+  const double *x_in, *u_in;
+  double *x_out, *u_out;
+  const double *V;          // per RVE
+  const int32_t *solve_rve;
+  const double *solve_F;    // 9 per solve
+  const int32_t *solve_flags;
+  const int32_t *order;     // blockIdx -> solve
+  double *sigma;            // 6 per solve
+  int32_t *status;          // per solve
+  int32_t *steps;           // per solve (diagnostic)
+  int32_t mode;
+  int32_t n_steps, max_steps, freq;
+  double eta, crit, tol;
+};
+
+// A kernel variant: threads per solve, free-node slots per thread and the
+// placement flags.
+enum : int32_t {
+  kEllInLds = 1,   // free-node incidence lists staged in LDS
+  kDoubleBuf = 2,  // positions double-buffered in LDS (one barrier per step)
+};
+
+struct Variant {
+  int32_t prec;    // MF_PREC_*
+  int32_t block;   // threads per workgroup
+  int32_t npt;     // free-node slots per thread
+  int32_t flags;
+};
+
+// All compiled variants (the planner picks from this list).
+const Variant *variants(int *count);
+// LDS bytes a variant needs for one RVE.
+size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries);
+// Largest dynamic LDS a workgroup may request.
+constexpr size_t kMaxLds = 160 * 1024;
+
+hipError_t launch_solve(const Variant &v, const SolveArgs &a, int n_blocks,
+                        size_t lds, hipStream_t s);
+hipError_t lds_probe(double *bytes_per_s);
+hipError_t hbm_probe(int64_t bytes, double *bytes_per_s);
+hipError_t launch_broadcast_F(const double *F, double *out, int n, hipStream_t s);
+hipError_t launch_update_volume(const double *F, int nF, double *V, int n_rve,
+                                hipStream_t s);
+hipError_t launch_fd(const double *sig_a, const double *sig_b, double *C,
+                     int n_rve, int column, double inv_h, int stride_a,
+                     int stride_b, hipStream_t s);
+hipError_t launch_fd_all(const double *sigma, double *C, int n_rve, int scheme,
+                         double h, hipStream_t s);
+hipError_t launch_orientation(const RveDev *rves, const double *x,
+                              const int32_t *fib, const double *normal,
+                              int n_normal, double *omega, int n_rve,
+                              hipStream_t s);
+
+}  // namespace mf

@@ -1,0 +1,376 @@
+// FP64 explicit solve with the reference's operation order.  This file is
+// compiled with -ffp-contract=off so every product and sum rounds as in the
+// reference C++ (BatchedFiberRVEAnalysisExplicitTeamOuterLoop.h:180-437,
+// BatchedFiberRVEAnalysisExplicitBase.h:155-287).
+#include "kernels_common.hpp"
+
+namespace mf {
+namespace kern {
+
+// ============================================================================
+// FP64 kernel
+// ============================================================================
+template <int B, int NPT, int FL>
+__global__ __launch_bounds__(B) void solve64(SolveArgs a) {
+#pragma clang fp contract(off)
+  constexpr bool kEll = (FL & kEllInLds) != 0;
+  constexpr bool kDbuf = (FL & kDoubleBuf) != 0;
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int64_t nb = R.node_off;
+
+  // LDS carve (mirrors lds_bytes)
+  size_t off = 0;
+  double *pos = reinterpret_cast<double *>(smem);
+  off += sizeof(double) * 3 * Np * (kDbuf ? 2 : 1);
+  double *red = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * 6 * Nbp;
+  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
+  off += align16(sizeof(ReactDev) * kMaxReactions);
+  double *l0s = reinterpret_cast<double *>(smem + off);
+  off += kEll ? sizeof(double) * R.ell_free_entries : 0;
+  int32_t *ws = reinterpret_cast<int32_t *>(smem + off);
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  if (kEll) {
+    for (int i = tid; i < R.ell_free_entries; i += B) {
+      ws[i] = gw[i];
+      l0s[i] = gl0[i];
+    }
+  }
+  const int32_t *Ew = kEll ? ws : gw;
+  const double *El0 = kEll ? l0s : gl0;
+  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  const bool lin_only =
+      R.n_react == 1 && a.reacts[R.react_off].type == MF_REACTION_LINEAR;
+  const Lin64 lin{a.reacts[R.react_off].E, a.reacts[R.react_off].A};
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  // ---- initial state -------------------------------------------------------
+  // relax: u += (F - I) x (applyIncrementalDeformationToDisplacement,
+  //        ...Base.h:44-54); ramp: boundary increment g = (F - I) x kept aside.
+  // x = X + u (getCurrentCoord, ...Base.h:192-202).
+  double xr[NPT][3], vr[NPT][3], fr[NPT][3], mr[NPT];
+  int degr[NPT], Dr[NPT], baser[NPT];
+  auto init_node = [&](int r, double (&x)[3]) {
+    const int64_t g = nb + r;
+    double xp[3], up[3], u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+    double inc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u[k] = ramp ? up[k] : inc[k] + up[k];
+      x[k] = a.X[3 * g + k] + u[k];
+      pos[k * Np + r] = x[k];
+      if (kDbuf) pos[3 * Np + k * Np + r] = x[k];
+    }
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = x[k];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    const int r = s * B + tid;
+    double x[3] = {0, 0, 0};
+    if (r < N) init_node(r, x);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xr[s][d] = x[d];
+      vr[s][d] = 0.0;
+      fr[s][d] = 0.0;
+    }
+    degr[s] = 0;
+    Dr[s] = 0;
+    baser[s] = 0;
+    mr[s] = 1.0;
+    if (r < Nf) {
+      mr[s] = a.mass[nb + r];
+      degr[s] = a.deg[nb + r];
+      const int k = r / kWave;
+      Dr[s] = a.slice_D[R.slice_off + k];
+      baser[s] = a.slice_base[R.slice_off + k] + lane;
+    }
+  }
+  for (int r = NPT * B + tid; r < N; r += B) {
+    double x[3];
+    init_node(r, x);
+  }
+  __syncthreads();
+
+  const double c_sound = R.c_sound;
+  int cur = 0;
+  // internal force of a free node at the positions in buffer `buf`
+  auto node_force = [&](int s, int buf, double (&f)[3]) {
+    const double *P = pos + (kDbuf ? buf * 3 * Np : 0);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int D = __builtin_amdgcn_readfirstlane(Dr[s]);
+    for (int j = 0; j < D; ++j) {
+      if (j < degr[s]) {
+        const int idx = baser[s] + j * kWave;
+        const int32_t wv = Ew[idx];
+        const double l0 = El0[idx];
+        const int w = wv & 0xFFFFFF;
+        const double dx = P[w] - xr[s][0];
+        const double dy = P[Np + w] - xr[s][1];
+        const double dz = P[2 * Np + w] - xr[s][2];
+        const double l = sqrt(dx * dx + dy * dy + dz * dz);
+        double frc;
+        if (lin_only) {
+          double lr = l / l0;
+          double gs = 1.0 / 2.0 * (lr * lr - 1);
+          frc = lr * lin.E * lin.A * gs;
+        } else {
+          frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+        }
+        const double t = frc / l;
+        fx = fx - dx * t;
+        fy = fy - dy * t;
+        fz = fz - dz * t;
+      }
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  // min fiber length over all fibers (getElementLengths, TeamOuterLoop.h:53-82)
+  auto min_length = [&](int buf) {
+    const double *P = pos + (kDbuf ? buf * 3 * Np : 0);
+    double lmin = 1.0e300;
+    const int32_t *fb = a.fib + 2 * R.fib_off;
+    for (int e = tid; e < M; e += B) {
+      const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+      double x1 = P[n2] - P[n1];
+      double x2 = P[Np + n2] - P[Np + n1];
+      double x3 = P[2 * Np + n2] - P[2 * Np + n1];
+      double l = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+      lmin = l < lmin ? l : lmin;
+    }
+    return lmin;
+  };
+
+  // ---- RunInitialization (TeamOuterLoop.h:215-289) ---------------------------
+  double fint[NPT][3];
+  double resid_local = 0.0;
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    if (s * B + tid < Nf) {
+      node_force(s, cur, fint[s]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double mass = mr[s] * a.eta;
+        double fdamp = mass * vr[s][d];
+        fr[s][d] = -fint[s][d] - fdamp;
+      }
+    }
+  }
+  double lmin = min_length(cur);
+  block_min_sum<B>(lmin, resid_local, red);
+  double dt_crit = lmin / c_sound;
+
+  // ---- main loop (TeamOuterLoop.h:340-371) -----------------------------------
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  for (;;) {
+    const double h = a.crit * dt_crit * 0.5;
+    const int nxt = kDbuf ? cur ^ 1 : 0;
+    if (!kDbuf) __syncthreads();  // all force reads of the last step done
+    double *Pn = pos + (kDbuf ? nxt * 3 * Np : 0);
+    // updates(): half kick + drift of free nodes (TeamOuterLoop.h:157-175)
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      const int r = s * B + tid;
+      if (r < Nf) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double vl = vr[s][d] + h * fr[s][d] / mr[s];
+          vr[s][d] = vl;
+          xr[s][d] = xr[s][d] + 2 * h * vl;
+          Pn[d * Np + r] = xr[s][d];
+        }
+      }
+    }
+    if (ramp) {
+      const double alpha = (double)(step + 1) / (double)a.n_steps;
+      for (int b = tid; b < Nb; b += B) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          Pn[d * Np + Nf + b] = bst[d * Nbp + b] + alpha * bst[(3 + d) * Nbp + b];
+      }
+    }
+    __syncthreads();
+    cur = nxt;
+    const bool refresh = (step % freq) == 0;
+    double rsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      if (s * B + tid < Nf) {
+        node_force(s, cur, fint[s]);
+        double rn = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double mass = mr[s] * a.eta;
+          double fdamp = mass * vr[s][d];
+          double lres = -fint[s][d];
+          fr[s][d] = lres - fdamp;
+          rn += lres * lres;
+        }
+        rsum += rn;
+      }
+    }
+    if (refresh) {
+      double lm = min_length(cur);
+      block_min_sum<B>(lm, rsum, red);
+      dt_crit = lm / c_sound;
+      residual = sqrt(rsum);
+    }
+    // updateAccelVels(): second half kick (TeamOuterLoop.h:144-156)
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      if (s * B + tid < Nf) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vr[s][d] += h * fr[s][d] / mr[s];
+      }
+    }
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // ---- FinalizeData + computeCauchyStress ------------------------------------
+  // Boundary internal forces gather every incident fiber (TeamOuterLoop.h:
+  // 374-434); stress = s/V * sum_b sym(x_b (x) f_b) (...Base.h:84-123).
+  const double *P = pos + (kDbuf ? cur * 3 * Np : 0);
+  __syncthreads();  // the last refresh reduction has been read by every wave
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const double alpha_end = ramp ? 1.0 : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    const int r = Nf + b;
+    double xb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xb[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const int k = r / kWave;
+    const int base = a.slice_base[R.slice_off + k] + (r & (kWave - 1));
+    const int dg = a.deg[nb + r];
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const double l0 = gl0[idx];
+      const int w = wv & 0xFFFFFF;
+      const double dx = P[w] - xb[0];
+      const double dy = P[Np + w] - xb[1];
+      const double dz = P[2 * Np + w] - xb[2];
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      const double t = frc / l;
+      f1 = f1 - dx * t;
+      f2 = f2 - dy * t;
+      f3 = f3 - dz * t;
+    }
+    st[0] += xb[0] * f1;
+    st[1] += xb[1] * f2;
+    st[2] += xb[2] * f3;
+    st[3] += 0.5 * (xb[1] * f3 + xb[2] * f2);
+    st[4] += 0.5 * (xb[0] * f3 + xb[2] * f1);
+    st[5] += 0.5 * (xb[0] * f2 + xb[1] * f1);
+    if (a.solve_flags[sid] & kWriteX) {
+      for (int d = 0; d < 3; ++d) a.x_out[3 * (nb + r) + d] = xb[d];
+    }
+    if (a.solve_flags[sid] & kWriteU) {
+      for (int d = 0; d < 3; ++d)
+        a.u_out[3 * (nb + r) + d] = xb[d] - a.X[3 * (nb + r) + d];
+    }
+  }
+  // free nodes: u = x - X (TeamOuterLoop.h:427-432)
+  const int flags = a.solve_flags[sid];
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) {
+      const int64_t g = nb + r;
+      if (flags & kWriteX)
+        for (int d = 0; d < 3; ++d) a.x_out[3 * g + d] = xr[s][d];
+      if (flags & kWriteU)
+        for (int d = 0; d < 3; ++d) a.u_out[3 * g + d] = xr[s][d] - a.X[3 * g + d];
+    }
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !(v == v) || v - v != 0.0;
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
+}  // namespace kern
+
+hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
+                  size_t lds, hipStream_t s) {
+#define MF_VARIANT_LAUNCH(P, B, N, F)                                        \
+  if constexpr (P == MF_PREC_FP64)                                           \
+    if (v.block == B && v.npt == N && v.flags == F) {                       \
+    auto k = kern::solve64<B, N, F>;                                          \
+    if (lds > 64 * 1024)                                                     \
+      (void)hipFuncSetAttribute((const void *)k,                             \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)kMaxLds);                               \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);               \
+    return hipGetLastError();                                                \
+  }
+  MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
+#undef MF_VARIANT_LAUNCH
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mf

@@ -1,0 +1,106 @@
+"""Host-side checks of libmumfim_amd.so that need no GPU: the C ABI exports,
+the seeded generator and the reference-format loader."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mumfim_amd as mf
+from conftest import ROOT, GOLDEN
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "mumfim_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = mf.lib()
+    names = header_functions()
+    assert len(names) > 30
+    for n in names:
+        assert hasattr(lib, n), n
+    # the Python signature table covers the whole header
+    assert sorted(mf.SIGNATURES) == names
+    assert lib.mf_version() >= 1
+
+
+def test_default_params_reproduce_reference_constants():
+    p = mf.default_params()
+    assert p.mode == mf.MF_MODE_RELAX and p.precision == mf.MF_PREC_FP64
+    assert p.dt_update_freq == 100  # UPDATE_FREQ, TeamOuterLoop.h:4
+    assert p.crit_time_scale_factor == 0.8  # TeamOuterLoop.h:312
+    assert p.tolerance == 1e-6  # TeamOuterLoop.h:371
+    assert p.fd_step == 1e-5  # BatchedFiberRVEAnalysisExplicit.h:331
+    assert p.visc_damp_coeff == 100.0  # fiber_only.yaml:16
+
+
+def _knn_bruteforce(X, k):
+    d = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d, np.inf)
+    edges = set()
+    for i in range(len(X)):
+        order = np.lexsort((np.arange(len(X)), d[i]))[:k]
+        for j in order:
+            edges.add((min(i, j), max(i, j)))
+    return np.array(sorted(edges), np.int32)
+
+
+@pytest.mark.parametrize("n,seed", [(60, 1), (200, 7), (500, 3)])
+def test_generator_is_exact_knn(n, seed):
+    net = mf.generate_network(seed=seed, n_nodes=n)
+    X = net.coords
+    assert X.shape == (n, 3) and X.min() >= -0.5 and X.max() < 0.5
+    ref = _knn_bruteforce(X, 4)
+    assert np.array_equal(net.fibers, ref)
+    band = 0.5 - np.abs(X).max(1) <= 0.02
+    assert np.array_equal(net.boundary, np.nonzero(band)[0])
+    deg = np.bincount(net.fibers.ravel(), minlength=n)
+    assert deg.min() >= 4
+    # ~2.5 fibers per node for the symmetric 4-NN graph
+    assert 2.0 < len(net.fibers) / n < 3.0
+
+
+def test_generator_is_deterministic_and_seeded():
+    a = mf.generate_network(seed=11, n_nodes=300, nonlinear_fraction=0.5)
+    b = mf.generate_network(seed=11, n_nodes=300, nonlinear_fraction=0.5)
+    c = mf.generate_network(seed=12, n_nodes=300, nonlinear_fraction=0.5)
+    assert np.array_equal(a.coords, b.coords) and np.array_equal(a.fibers, b.fibers)
+    assert np.array_equal(a.fiber_reaction, b.fiber_reaction)
+    assert not np.array_equal(a.coords, c.coords)
+    frac = a.fiber_reaction.mean()
+    assert 0.4 < frac < 0.6
+    assert a.reactions[1].type == mf.MF_REACTION_NONLINEAR
+    assert a.reactions[1].B == 1.2 and a.reactions[1].length_ratio_trns == 1.05
+
+
+def test_uniform_stream():
+    u = mf.uniform(5, 9, 1000)
+    assert np.array_equal(u, mf.uniform(5, 9, 1000))
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.05
+    assert not np.array_equal(u, mf.uniform(5, 10, 1000))
+
+
+def test_load_reference_network():
+    net = mf.load_network(os.path.join(GOLDEN, "network_2_7", "Network_2_7.txt"))
+    assert (net.n_nodes, net.n_fibers) == (336, 635)  # Network_2_7.txt.summary
+    assert len(net.boundary) == 261
+    r = net.reactions[0]
+    assert (r.type, r.radius, r.E, r.density) == (0, 5e-5, 1e8, 1000.0)
+    assert np.all(net.fiber_reaction == 0)
+    on = (np.abs(np.abs(net.coords) - 0.5) < 1e-8).any(1)
+    assert np.array_equal(net.boundary, np.nonzero(on)[0])
+
+
+def test_load_errors_are_reported():
+    with pytest.raises(mf.MfError):
+        mf.load_network(os.path.join(GOLDEN, "does_not_exist.txt"))
+
+
+def test_batch_create_rejects_bad_input_before_touching_the_gpu():
+    net = mf.generate_network(seed=1, n_nodes=50)
+    bad = mf.Network(net.coords, np.array([[0, 0]], np.int32), net.boundary, net.reactions)
+    with pytest.raises(mf.MfError):
+        mf.BatchedFiberRVEAnalysisExplicit([bad])
