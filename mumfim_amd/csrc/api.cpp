@@ -86,9 +86,9 @@ void make_react_dev(const Reaction &q, ReactDev &d) {
     ext(q.lt, d.Kt, d.bt);
     ext(q.lc, d.Kc, d.bc);
     d.lt_f = (float)q.lt;
-    d.lt2 = (float)(q.lt * q.lt);
+    d.lt2m1 = (float)(q.lt * q.lt - 1.0);
     d.lc_f = (float)q.lc;
-    d.lc2 = (float)(q.lc * q.lc);
+    d.lc2m1 = (float)(q.lc * q.lc - 1.0);
   }
 }
 

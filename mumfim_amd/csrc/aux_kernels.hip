@@ -157,7 +157,8 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
   if (v.prec == MF_PREC_FP64) {
     b += sizeof(double) * 3 * Np * ((v.flags & kDoubleBuf) ? 2 : 1);
   } else {
-    b += 16 * 2 * Np;
+    // displacement buffers + reference coordinates
+    b += 16 * Np * ((v.flags & kDoubleBuf) ? 3 : 2);
   }
   b += sizeof(double) * NW * kern::kRedStride;
   b += sizeof(double) * 6 * Nbp;

@@ -120,19 +120,23 @@ __device__ inline double react64(const ReactDev &q, double l0, double l) {
   return nl_force(alpha, q.B, lr, g);
 }
 
-// ---- reaction laws, FP32 algebraic forms: t = f/l from lambda^2 ---------
-__device__ inline float react32_t(const ReactDev &q, float lam2, float il) {
-  if (q.type == MF_REACTION_LINEAR) return q.EA_half * il * (lam2 - 1.0f);
-  if (lam2 > q.lt2) {
-    float ilam = __frsqrt_rn(lam2);
-    float lam = lam2 * ilam;
+// ---- reaction laws, FP32 algebraic forms --------------------------------
+// t = f/l as a function of e2 = lambda^2 - 1 = (l^2 - l0^2)/l0^2, which the
+// FP32 kernel forms without cancellation from the displacement difference.
+__device__ inline float react32_t(const ReactDev &q, float e2, float il) {
+  if (q.type == MF_REACTION_LINEAR) return q.EA_half * il * e2;
+  if (e2 > q.lt2m1) {
+    const float lam2 = 1.0f + e2;
+    const float ilam = __frsqrt_rn(lam2);
+    const float lam = lam2 * ilam;
     return (q.Kt * (lam - q.lt_f) + q.bt) * il * ilam;
-  } else if (lam2 < q.lc2) {
-    float ilam = __frsqrt_rn(lam2);
-    float lam = lam2 * ilam;
+  } else if (e2 < q.lc2m1) {
+    const float lam2 = 1.0f + e2;
+    const float ilam = __frsqrt_rn(lam2);
+    const float lam = lam2 * ilam;
     return (q.Kc * (lam - q.lc_f) + q.bc) * il * ilam;
   }
-  return q.alpha * il * expm1f(q.B_half * (lam2 - 1.0f));
+  return q.alpha * il * expm1f(q.B_half * e2);
 }
 
 // Fast path when the whole RVE uses one linear reaction.
@@ -156,7 +160,9 @@ struct Lin64 {
   X(MF_PREC_FP32C, 256, 1, 3) X(MF_PREC_FP32C, 256, 1, 2)                    \
   X(MF_PREC_FP32C, 256, 2, 3) X(MF_PREC_FP32C, 256, 2, 2)                    \
   X(MF_PREC_FP32C, 256, 4, 3) X(MF_PREC_FP32C, 256, 4, 2)                    \
-  X(MF_PREC_FP32C, 512, 2, 3) X(MF_PREC_FP32C, 512, 2, 2)                    \
-  X(MF_PREC_FP32C, 512, 4, 3) X(MF_PREC_FP32C, 512, 4, 2)                    \
-  X(MF_PREC_FP32C, 1024, 2, 2) X(MF_PREC_FP32C, 1024, 4, 2)
+  X(MF_PREC_FP32C, 256, 4, 0) X(MF_PREC_FP32C, 512, 2, 3)                    \
+  X(MF_PREC_FP32C, 512, 2, 2) X(MF_PREC_FP32C, 512, 4, 3)                    \
+  X(MF_PREC_FP32C, 512, 4, 2) X(MF_PREC_FP32C, 512, 4, 0)                    \
+  X(MF_PREC_FP32C, 1024, 2, 2) X(MF_PREC_FP32C, 1024, 2, 0)                  \
+  X(MF_PREC_FP32C, 1024, 4, 2) X(MF_PREC_FP32C, 1024, 4, 0)
 

@@ -112,14 +112,14 @@ struct ReactDev {
   int32_t pad_;
   // FP64 reference-formula inputs.
   double E, A, B, lt, lc;
-  // FP32 algebraic forms: t = f/l as a function of lambda^2.
-  float EA_half;   // linear: t = EA/2 * il * (lam2 - 1)
+  // FP32 algebraic forms: t = f/l as a function of e2 = lambda^2 - 1.
+  float EA_half;   // linear: t = EA/2 * il * e2
   float alpha;     // nonlinear alpha = E*A/B
   float B_half;    // B/2
-  float lt2;       // lambda_t^2
+  float lt2m1;     // lambda_t^2 - 1
   float lt_f;
   float Kt, bt;    // tension extension: f = Kt*(lam - lt) + bt
-  float lc_f, lc2; // compression transition lambda_c, lambda_c^2
+  float lc_f, lc2m1; // compression transition lambda_c, lambda_c^2 - 1
   float Kc, bc;    // compression extension: f = Kc*(lam - lc) + bc
 };
 

@@ -14,9 +14,17 @@ __device__ inline void two_sum(float a, float b, float &s, float &e) {
   e = (a - (s - bp)) + (b - bp);
 }
 
+// State per free node: displacement u = (hi, lo) in FP32 advanced with
+// TwoSum, velocity and acceleration in FP32.  LDS holds the displacement hi
+// parts (double-buffered) and the reference coordinates X (FP32).  A fiber
+// seen from node v towards w has reference vector d0 = X_w - X_v,
+// displacement difference delta = u_w - u_v, current vector d = d0 + delta
+// and l^2 - l0^2 = (d0 + d) . delta, which keeps full relative precision at
+// small strain (the direct form d.d - l0^2 loses eps/strain).
 template <int B, int NPT, int FL>
 __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   constexpr bool kEll = (FL & kEllInLds) != 0;
+  constexpr bool kDbuf = (FL & kDoubleBuf) != 0;
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
@@ -28,11 +36,13 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   const int64_t nb = R.node_off;
 
   size_t off = 0;
-  float4 *pos = reinterpret_cast<float4 *>(smem);  // [2][Np]
-  off += sizeof(float4) * 2 * Np;
+  float4 *pos = reinterpret_cast<float4 *>(smem);  // [1 or 2][Np] displacement hi
+  off += sizeof(float4) * (kDbuf ? 2 : 1) * Np;
+  float4 *Xs = reinterpret_cast<float4 *>(smem + off);  // [Np] reference X
+  off += sizeof(float4) * Np;
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);
+  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
   off += sizeof(double) * 6 * Nbp;
   ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
   off += align16(sizeof(ReactDev) * kMaxReactions);
@@ -60,9 +70,11 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
 
-  float xh[NPT][3], xl[NPT][3], vr[NPT][3], ar[NPT][3], imr[NPT];
+  float Xv[NPT][3], uh[NPT][3], ul[NPT][3], vr[NPT][3], ar[NPT][3], imr[NPT];
   int degr[NPT], Dr[NPT], baser[NPT];
-  auto init_node = [&](int r, double (&x)[3]) {
+  // initial displacement: relax adds the affine increment (F - I) x to u,
+  // ramp keeps u and sets the boundary increment aside.
+  auto init_node = [&](int r, double (&u)[3]) {
     const int64_t g = nb + r;
     double xp[3], up[3];
 #pragma unroll
@@ -76,23 +88,18 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
                (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
                (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
-    float4 p;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double u = ramp ? up[k] : inc[k] + up[k];
-      x[k] = a.X[3 * g + k] + u;
-    }
-    p.x = (float)x[0];
-    p.y = (float)x[1];
-    p.z = (float)x[2];
-    p.w = 0.0f;
+    for (int k = 0; k < 3; ++k) u[k] = ramp ? up[k] : inc[k] + up[k];
+    const float4 p = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
     pos[r] = p;
-    pos[Np + r] = p;
+    if (kDbuf) pos[Np + r] = p;
+    Xs[r] = make_float4((float)a.X[3 * g], (float)a.X[3 * g + 1],
+                        (float)a.X[3 * g + 2], 0.0f);
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[k * Nbp + b] = x[k];
+        bst[k * Nbp + b] = u[k];
         bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
       }
     }
@@ -101,12 +108,13 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 #pragma unroll
   for (int s = 0; s < NPT; ++s) {
     const int r = s * B + tid;
-    double x[3] = {0, 0, 0};
-    if (r < N) init_node(r, x);
+    double u[3] = {0, 0, 0};
+    if (r < N) init_node(r, u);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      xh[s][d] = (float)x[d];
-      xl[s][d] = (float)(x[d] - (double)xh[s][d]);
+      uh[s][d] = (float)u[d];
+      ul[s][d] = (float)(u[d] - (double)uh[s][d]);
+      Xv[s][d] = r < N ? (float)a.X[3 * (nb + r) + d] : 0.0f;
       vr[s][d] = 0.0f;
       ar[s][d] = 0.0f;
     }
@@ -123,15 +131,15 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     }
   }
   for (int r = NPT * B + tid; r < N; r += B) {
-    double x[3];
-    init_node(r, x);
+    double u[3];
+    init_node(r, u);
   }
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
   int cur = 0;
   auto node_force = [&](int s, int buf, float (&f)[3]) {
-    const float4 *P = pos + buf * Np;
+    const float4 *P = pos + (kDbuf ? buf * Np : 0);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int D = __builtin_amdgcn_readfirstlane(Dr[s]);
     for (int j = 0; j < D; ++j) {
@@ -139,14 +147,19 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const int idx = baser[s] + j * kWave;
         const int32_t wv = Ew[idx];
         const float il = Eil[idx];
-        const float4 q = P[wv & 0xFFFFFF];
-        const float dx = (q.x - xh[s][0]) - xl[s][0];
-        const float dy = (q.y - xh[s][1]) - xl[s][1];
-        const float dz = (q.z - xh[s][2]) - xl[s][2];
-        const float l2 = dx * dx + dy * dy + dz * dz;
-        const float lam2 = l2 * (il * il);
-        const float t = lin_only ? lin_EA_half * il * (lam2 - 1.0f)
-                                 : react32_t(rxs[(uint32_t)wv >> 24], lam2, il);
+        const int w = wv & 0xFFFFFF;
+        const float4 q = P[w];
+        const float4 xq = Xs[w];
+        const float ex = xq.x - Xv[s][0], ey = xq.y - Xv[s][1], ez = xq.z - Xv[s][2];
+        const float gx = (q.x - uh[s][0]) - ul[s][0];
+        const float gy = (q.y - uh[s][1]) - ul[s][1];
+        const float gz = (q.z - uh[s][2]) - ul[s][2];
+        const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+        // l^2 - l0^2 = (d0 + d) . delta
+        const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+        const float e2 = sl * (il * il);
+        const float t = lin_only ? lin_EA_half * il * e2
+                                 : react32_t(rxs[(uint32_t)wv >> 24], e2, il);
         fx -= dx * t;
         fy -= dy * t;
         fz -= dz * t;
@@ -157,12 +170,15 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     f[2] = fz;
   };
   auto min_length2 = [&](int buf) {
-    const float4 *P = pos + buf * Np;
+    const float4 *P = pos + (kDbuf ? buf * Np : 0);
     float lmin = 3.0e38f;
     const int32_t *fb = a.fib + 2 * R.fib_off;
     for (int e = tid; e < M; e += B) {
-      const float4 p1 = P[fb[2 * e]], p2 = P[fb[2 * e + 1]];
-      float x1 = p2.x - p1.x, x2 = p2.y - p1.y, x3 = p2.z - p1.z;
+      const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+      const float4 p1 = P[n1], p2 = P[n2], X1 = Xs[n1], X2 = Xs[n2];
+      const float x1 = (X2.x - X1.x) + (p2.x - p1.x);
+      const float x2 = (X2.y - X1.y) + (p2.y - p1.y);
+      const float x3 = (X2.z - X1.z) + (p2.z - p1.z);
       lmin = fminf(lmin, x1 * x1 + x2 * x2 + x3 * x3);
     }
     return (double)lmin;
@@ -191,43 +207,36 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   for (;;) {
     const float h = crit_half * dt_crit;
     const float h2 = 2.0f * h;
-    const int nxt = cur ^ 1;
+    const int nxt = kDbuf ? cur ^ 1 : 0;
     float4 *Pn = pos + nxt * Np;
+    if (!kDbuf) __syncthreads();  // all force reads of the last step done
+    // half kick + drift; u (hi, lo) += 2h v with TwoSum
 #pragma unroll
     for (int s = 0; s < NPT; ++s) {
       const int r = s * B + tid;
       if (r < Nf) {
-        float4 p;
         float pc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float vl = vr[s][d] + h * ar[s][d];
+          const float vl = vr[s][d] + h * ar[s][d];
           vr[s][d] = vl;
-          // compensated position update: (hi, lo) += 2h*v
-          float y = xl[s][d] + h2 * vl;
+          const float y = ul[s][d] + h2 * vl;
           float sh, e;
-          two_sum(xh[s][d], y, sh, e);
-          xh[s][d] = sh;
-          xl[s][d] = e;
+          two_sum(uh[s][d], y, sh, e);
+          uh[s][d] = sh;
+          ul[s][d] = e;
           pc[d] = sh;
         }
-        p.x = pc[0];
-        p.y = pc[1];
-        p.z = pc[2];
-        p.w = 0.0f;
-        Pn[r] = p;
+        Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
       }
     }
     if (ramp) {
       const double alpha = (double)(step + 1) / (double)a.n_steps;
-      for (int b = tid; b < Nb; b += B) {
-        float4 p;
-        p.x = (float)(bst[b] + alpha * bst[3 * Nbp + b]);
-        p.y = (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]);
-        p.z = (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]);
-        p.w = 0.0f;
-        Pn[Nf + b] = p;
-      }
+      for (int b = tid; b < Nb; b += B)
+        Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
+                                 (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
+                                 (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
+                                 0.0f);
     }
     __syncthreads();
     cur = nxt;
@@ -272,41 +281,41 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   }
   if (residual != residual) status = MF_STATUS_NAN;
 
-  // finalize: publish the low parts so boundary gathers see full positions
-  float4 *Ph = pos + cur * Np;
-  float4 *Pl = pos + (cur ^ 1) * Np;
-  __syncthreads();  // every read of the other buffer is done
+  // finalize: with two buffers the spare one publishes the low parts so the
+  // boundary gathers see full displacements; with one, the hi parts only.
+  float4 *Ph = pos + (kDbuf ? cur * Np : 0);
+  float4 *Pl = pos + (kDbuf ? (cur ^ 1) * Np : 0);
+  __syncthreads();  // every read of the other buffer and of red is done
+  if (kDbuf) {
 #pragma unroll
-  for (int s = 0; s < NPT; ++s) {
-    const int r = s * B + tid;
-    if (r < Nf) {
-      float4 p;
-      p.x = xl[s][0];
-      p.y = xl[s][1];
-      p.z = xl[s][2];
-      p.w = 0.0f;
-      Pl[r] = p;
+    for (int s = 0; s < NPT; ++s) {
+      const int r = s * B + tid;
+      if (r < Nf) Pl[r] = make_float4(ul[s][0], ul[s][1], ul[s][2], 0.0f);
     }
   }
   const double alpha_end = ramp ? 1.0 : 0.0;
   for (int b = tid; b < Nb; b += B) {
-    float4 p;
-    double xb[3];
-    for (int d = 0; d < 3; ++d) xb[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-    p.x = (float)(xb[0] - (double)Ph[Nf + b].x);
-    p.y = (float)(xb[1] - (double)Ph[Nf + b].y);
-    p.z = (float)(xb[2] - (double)Ph[Nf + b].z);
-    p.w = 0.0f;
-    Pl[Nf + b] = p;
+    double ub[3];
+    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+    Ph[Nf + b] = hb;
+    if (kDbuf)
+      Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
+                               (float)(ub[2] - (double)hb.z), 0.0f);
   }
   __syncthreads();
   double st[6] = {0, 0, 0, 0, 0, 0};
   const int flags = a.solve_flags[sid];
   for (int b = tid; b < Nb; b += B) {
     const int r = Nf + b;
-    double xb[3];
-    for (int d = 0; d < 3; ++d) xb[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-    const float4 hb = Ph[r], lb = Pl[r];
+    const int64_t gr = nb + r;
+    double ub[3], xb[3];
+    for (int d = 0; d < 3; ++d) {
+      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+      xb[d] = a.X[3 * gr + d] + ub[d];
+    }
+    const float4 hb = Ph[r], Xb = Xs[r];
+    const float4 lb = kDbuf ? Pl[r] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int k = r / kWave;
     const int base = a.slice_base[R.slice_off + k] + (r & (kWave - 1));
     const int dg = a.deg[nb + r];
@@ -316,14 +325,17 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const int32_t wv = gw[idx];
       const float il = gil[idx];
       const int w = wv & 0xFFFFFF;
-      const float4 qh = Ph[w], ql = Pl[w];
-      const float dx = (qh.x - hb.x) + (ql.x - lb.x);
-      const float dy = (qh.y - hb.y) + (ql.y - lb.y);
-      const float dz = (qh.z - hb.z) + (ql.z - lb.z);
-      const float l2 = dx * dx + dy * dy + dz * dz;
-      const float lam2 = l2 * (il * il);
-      const float t = lin_only ? lin_EA_half * il * (lam2 - 1.0f)
-                               : react32_t(rxs[(uint32_t)wv >> 24], lam2, il);
+      const float4 qh = Ph[w], xq = Xs[w];
+      const float4 ql = kDbuf ? Pl[w] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float ex = xq.x - Xb.x, ey = xq.y - Xb.y, ez = xq.z - Xb.z;
+      const float gx = (qh.x - hb.x) + (ql.x - lb.x);
+      const float gy = (qh.y - hb.y) + (ql.y - lb.y);
+      const float gz = (qh.z - hb.z) + (ql.z - lb.z);
+      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float e2 = sl * (il * il);
+      const float t = lin_only ? lin_EA_half * il * e2
+                               : react32_t(rxs[(uint32_t)wv >> 24], e2, il);
       f1 -= dx * t;
       f2 -= dy * t;
       f3 -= dz * t;
@@ -336,10 +348,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
     st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
     if (flags & kWriteX)
-      for (int d = 0; d < 3; ++d) a.x_out[3 * (nb + r) + d] = xb[d];
+      for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
     if (flags & kWriteU)
-      for (int d = 0; d < 3; ++d)
-        a.u_out[3 * (nb + r) + d] = xb[d] - a.X[3 * (nb + r) + d];
+      for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
   }
 #pragma unroll
   for (int s = 0; s < NPT; ++s) {
@@ -347,9 +358,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     if (r < Nf) {
       const int64_t g = nb + r;
       for (int d = 0; d < 3; ++d) {
-        double x = (double)xh[s][d] + (double)xl[s][d];
-        if (flags & kWriteX) a.x_out[3 * g + d] = x;
-        if (flags & kWriteU) a.u_out[3 * g + d] = x - a.X[3 * g + d];
+        const double u = (double)uh[s][d] + (double)ul[s][d];
+        if (flags & kWriteX) a.x_out[3 * g + d] = a.X[3 * g + d] + u;
+        if (flags & kWriteU) a.u_out[3 * g + d] = u;
       }
     }
   }
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     for (int i = 0; i < 6; ++i) {
       double v = st[i] * sv;
       a.sigma[6 * (int64_t)sid + i] = v;
-      bad = bad || !(v == v) || v - v != 0.0;
+      bad = bad || !isfinite(v);
     }
     if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
     a.status[sid] = status;

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
     for (int i = 0; i < 6; ++i) {
       double v = st[i] * sv;
       a.sigma[6 * (int64_t)sid + i] = v;
-      bad = bad || !(v == v) || v - v != 0.0;
+      bad = bad || !isfinite(v);
     }
     if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
     a.status[sid] = status;

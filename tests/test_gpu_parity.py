@@ -122,7 +122,9 @@ def test_cfg1_ramp_fp64_and_fp32c():
     s_g = g.run(F)
     e64 = rel_err(s_g, s_o)
     g32 = mf.BatchedFiberRVEAnalysisExplicit([net], ramp_params(precision=mf.MF_PREC_FP32C))
-    e32 = rel_err(g32.run(F), s_o)
+    s32, st32 = g32.run(F, return_status=True)
+    assert (st32 == 0).all()
+    e32 = rel_err(s32, s_o)
     say(f"  fp64 {e64:.3e}  fp32c {e32:.3e}")
     assert e64 <= FP64_TOL
     assert e32 <= FP32C_TOL
@@ -139,7 +141,9 @@ def test_cfg2_subset_fp64_and_fp32c():
     s_g = g.run(F)
     e64 = rel_err(s_g, s_o)
     g32 = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(precision=mf.MF_PREC_FP32C))
-    e32 = rel_err(g32.run(F), s_o)
+    s32, st32 = g32.run(F, return_status=True)
+    assert (st32 == 0).all()
+    e32 = rel_err(s32, s_o)
     say(f"  fp64 {e64:.3e}  fp32c {e32:.3e}")
     assert e64 <= FP64_TOL
     assert e32 <= FP32C_TOL
@@ -165,7 +169,9 @@ def test_uneven_mixed_reactions():
     assert (st == 0).all()
     e64 = rel_err(s_g, s_o)
     g32 = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(precision=mf.MF_PREC_FP32C, **p))
-    e32 = rel_err(g32.run(F), s_o)
+    s32, st32 = g32.run(F, return_status=True)
+    assert (st32 == 0).all()
+    e32 = rel_err(s32, s_o)
     say(f"  fp64 {e64:.3e}  fp32c {e32:.3e}")
     assert e64 <= FP64_TOL
     assert e32 <= FP32C_TOL
