@@ -48,20 +48,19 @@ def algorithmic_bytes(g, prec):
     return float((80 * Nf + 24 * Nb + 20 * M).sum())
 
 
-def time_solver(g, F_dev, sig_dev, steps, warmup, torch):
+def time_solver(g, F_dev, sig_dev, steps, warmup):
     import mumfim_amd as mf
-    R = g.n
     for _ in range(warmup):
         g.run_device(F_dev, sig_dev)
         g.synchronize()
     kern_ms = []
-    torch.cuda.synchronize()
+    mf.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         g.run_device(F_dev, sig_dev)
         g.synchronize()
         kern_ms.append(g.last_solve_ms()[0])
-    torch.cuda.synchronize()
+    mf.device_synchronize()
     t1 = time.perf_counter()
     return (t1 - t0), kern_ms
 
@@ -77,13 +76,14 @@ def cpu_baseline(nets_fn, F, n_steps, target_s=12.0):
     # calibrate with one RVE on one thread-slot
     o1 = Oracle(nets_fn(1), p)
     t = time.perf_counter()
-    o1.run(F[:1])
+    o1.run(F[:1], update_coords=False)
     t1 = time.perf_counter() - t
     n = int(max(threads, min(512, round(target_s / max(t1, 1e-3) * threads))))
     n = max(threads, (n // threads) * threads)
     o = Oracle(nets_fn(n), p)
     t = time.perf_counter()
-    s = o.run(F[:n])
+    # trial runs (update_coords=False) like the timed GPU batches: sigma / V0
+    s = o.run(F[:n], update_coords=False)
     dt = time.perf_counter() - t
     return n / dt, n, threads, dt, s
 
@@ -106,14 +106,13 @@ def main():
               "supported (RVEs shard by RVE across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import mumfim_amd as mf
 
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])
-    dev = torch.device("cuda:0")
-    F_dev = torch.from_numpy(F.reshape(R, 9).copy()).to(dev)
+    # inputs resident in HBM, allocated by the library's own HIP runtime
+    F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
     out = {}
     solvers = {}
     for name, prec in (("fp32c", mf.MF_PREC_FP32C), ("fp64", mf.MF_PREC_FP64)):
@@ -123,13 +122,13 @@ def main():
         p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, precision=prec)
         g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R))
         log(f"[bench] {name}: batch of {R} RVEs built in {time.perf_counter() - t:.1f}s")
-        sig = torch.zeros((R, 6), dtype=torch.float64, device=dev)
+        sig = mf.DeviceArray((R, 6), np.float64)
         steps = args.steps if name == "fp32c" else (args.fp64_steps or args.steps)
-        wall, kms = time_solver(g, F_dev, sig, steps, args.warmup, torch)
+        wall, kms = time_solver(g, F_dev, sig, steps, args.warmup)
         log(f"[bench] {name}: {steps} batches in {wall:.3f}s, kernel ms {kms}")
         solvers[name] = g
         out[name] = dict(wall=wall, steps=steps, kernel_ms=kms,
-                         sigma=sig.cpu().numpy(),
+                         sigma=sig.numpy(),
                          bytes=algorithmic_bytes(g, name))
     head = "fp32c" if "fp32c" in out else "fp64"
     h = out[head]

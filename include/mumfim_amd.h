@@ -185,6 +185,15 @@ int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
  * 4 ints per RVE.  Returns the number of distinct launch groups. */
 int mf_batch_plan(mf_batch *b, int32_t *plan);
 
+/* ---- device memory (no reference counterpart) ----------------------------- */
+/* Plain HBM buffers for the *_device entry points, allocated by the same HIP
+ * runtime the library runs on. */
+int mf_device_alloc(void **ptr, int64_t bytes);
+int mf_device_free(void *ptr);
+int mf_memcpy_to_device(void *dst, const void *src, int64_t bytes);
+int mf_memcpy_to_host(void *dst, const void *src, int64_t bytes);
+int mf_device_synchronize(void);
+
 /* ---- diagnostics (no reference counterpart) ------------------------------ */
 /* Measured ceilings for the roofline: LDS read bandwidth (ds_read_b128 over
  * every CU) and HBM copy bandwidth (read + write bytes), in bytes/s. */

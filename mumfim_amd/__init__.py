@@ -93,6 +93,11 @@ SIGNATURES = {
     "mf_generate_network": (C.c_int, [C.POINTER(mf_gen_params), _IP, _IP, _IP, _D, _IP, _IP, _IP, C.POINTER(mf_reaction)]),
     "mf_batch_create_generated": (C.c_int, [C.POINTER(mf_gen_params), _I, _IP, C.POINTER(mf_solver_params), C.POINTER(_P)]),
     "mf_uniform": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _D]),
+    "mf_device_alloc": (C.c_int, [C.POINTER(_P), C.c_int64]),
+    "mf_device_free": (C.c_int, [_P]),
+    "mf_memcpy_to_device": (C.c_int, [_P, _P, C.c_int64]),
+    "mf_memcpy_to_host": (C.c_int, [_P, _P, C.c_int64]),
+    "mf_device_synchronize": (C.c_int, []),
     "mf_measure_lds_bandwidth": (C.c_int, [_D]),
     "mf_measure_hbm_bandwidth": (C.c_int, [C.c_int64, _D]),
     "mf_load_network": (C.c_int, [C.c_char_p, _IP, _IP, _IP, _IP, _I, _D, _IP, _IP, _IP, C.POINTER(mf_reaction)]),
@@ -247,6 +252,57 @@ def uniform(seed, stream, n):
     return out
 
 
+class DeviceArray:
+    """An HBM buffer owned by the library's HIP runtime (float64, float32 or
+    int32), with host copies in and out."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().mf_device_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.shape, a.dtype)
+        d.copy_from(a)
+        return d
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().mf_memcpy_to_device(self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().mf_memcpy_to_host(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
+    def data_ptr(self):
+        return self.ptr
+
+    def free(self):
+        if self.ptr:
+            lib().mf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    check(lib().mf_device_synchronize())
+
+
 def measure_lds_bandwidth():
     v = C.c_double()
     check(lib().mf_measure_lds_bandwidth(C.byref(v)))
@@ -360,7 +416,7 @@ class BatchedFiberRVEAnalysisExplicit:
                 "Nb": z[:, 2].astype(np.int64), "M": z[:, 3].astype(np.int64)}
 
     def run_device(self, F_dev, sigma_dev, update_coords=False, status_dev=None):
-        """Queues run() on device buffers (torch tensors or raw pointers);
+        """Queues run() on device buffers (DeviceArray or raw pointers);
         F_dev holds 1 or n_rve row-major 3x3 matrices in float64."""
         fp = F_dev.data_ptr() if hasattr(F_dev, "data_ptr") else int(F_dev)
         sp = sigma_dev.data_ptr() if hasattr(sigma_dev, "data_ptr") else int(sigma_dev)

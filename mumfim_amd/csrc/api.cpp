@@ -267,6 +267,15 @@ int upload_batch(mf_batch *b) {
     d.n_react = (int)p.reactions.size();
     d.c_sound = p.c_sound;
     d.scale = p.scale;
+    {
+      // largest power of two with max|X| * scale < 2^30
+      double xm = 0;
+      for (double v : p.X) xm = std::max(xm, std::fabs(v));
+      int e = 0;
+      std::frexp(std::max(xm, 1e-300), &e);  // xm < 2^e
+      d.xq_scale = std::ldexp(1.0, 30 - e);
+      d.xq_inv = (float)std::ldexp(1.0, e - 30);
+    }
     X.insert(X.end(), p.X.begin(), p.X.end());
     mass.insert(mass.end(), p.mass.begin(), p.mass.end());
     deg.insert(deg.end(), p.deg.begin(), p.deg.end());
@@ -1165,6 +1174,36 @@ int mf_generate_network(const mf_gen_params *g, int32_t *n_nodes,
       r.length_ratio_trns = q.lt;
       r.cmp_ratio_trns = q.lc;
     }
+  return MF_OK;
+}
+
+int mf_device_alloc(void **ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return MF_ERR_INVALID;
+  *ptr = nullptr;
+  if (bytes == 0) return MF_OK;
+  MF_HIP(hipMalloc(ptr, (size_t)bytes));
+  return MF_OK;
+}
+
+int mf_device_free(void *ptr) {
+  if (ptr) MF_HIP(hipFree(ptr));
+  return MF_OK;
+}
+
+int mf_memcpy_to_device(void *dst, const void *src, int64_t bytes) {
+  if ((!dst || !src) && bytes) return MF_ERR_INVALID;
+  if (bytes) MF_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return MF_OK;
+}
+
+int mf_memcpy_to_host(void *dst, const void *src, int64_t bytes) {
+  if ((!dst || !src) && bytes) return MF_ERR_INVALID;
+  if (bytes) MF_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+int mf_device_synchronize(void) {
+  MF_HIP(hipDeviceSynchronize());
   return MF_OK;
 }
 

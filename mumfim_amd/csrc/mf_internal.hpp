@@ -104,6 +104,11 @@ struct RveDev {
   int32_t n_react;
   int32_t pad_;
   double c_sound, scale;
+  // Reference coordinates in fixed point for the FP32 kernel: X_q =
+  // rint(X * xq_scale) fits int32; xq_scale is a power of two.
+  double xq_scale;
+  float xq_inv;
+  int32_t pad2_;
 };
 
 // Reaction constants as the kernels consume them.

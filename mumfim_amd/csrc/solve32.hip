@@ -38,8 +38,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   size_t off = 0;
   float4 *pos = reinterpret_cast<float4 *>(smem);  // [1 or 2][Np] displacement hi
   off += sizeof(float4) * (kDbuf ? 2 : 1) * Np;
-  float4 *Xs = reinterpret_cast<float4 *>(smem + off);  // [Np] reference X
-  off += sizeof(float4) * Np;
+  int4 *Xs = reinterpret_cast<int4 *>(smem + off);  // [Np] reference X, fixed point
+  off += sizeof(int4) * Np;
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
   double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
@@ -70,7 +70,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
 
-  float Xv[NPT][3], uh[NPT][3], ul[NPT][3], vr[NPT][3], ar[NPT][3], imr[NPT];
+  int Xv[NPT][3];
+  float uh[NPT][3], ul[NPT][3], vr[NPT][3], ar[NPT][3], imr[NPT];
   int degr[NPT], Dr[NPT], baser[NPT];
   // initial displacement: relax adds the affine increment (F - I) x to u,
   // ramp keeps u and sets the boundary increment aside.
@@ -93,8 +94,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     const float4 p = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
     pos[r] = p;
     if (kDbuf) pos[Np + r] = p;
-    Xs[r] = make_float4((float)a.X[3 * g], (float)a.X[3 * g + 1],
-                        (float)a.X[3 * g + 2], 0.0f);
+    Xs[r] = make_int4((int)rint(a.X[3 * g] * R.xq_scale),
+                      (int)rint(a.X[3 * g + 1] * R.xq_scale),
+                      (int)rint(a.X[3 * g + 2] * R.xq_scale), 0);
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     for (int d = 0; d < 3; ++d) {
       uh[s][d] = (float)u[d];
       ul[s][d] = (float)(u[d] - (double)uh[s][d]);
-      Xv[s][d] = r < N ? (float)a.X[3 * (nb + r) + d] : 0.0f;
+      Xv[s][d] = r < N ? (int)rint(a.X[3 * (nb + r) + d] * R.xq_scale) : 0;
       vr[s][d] = 0.0f;
       ar[s][d] = 0.0f;
     }
@@ -137,6 +139,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
+  const float xq_inv = R.xq_inv;
   int cur = 0;
   auto node_force = [&](int s, int buf, float (&f)[3]) {
     const float4 *P = pos + (kDbuf ? buf * Np : 0);
@@ -149,8 +152,11 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const float il = Eil[idx];
         const int w = wv & 0xFFFFFF;
         const float4 q = P[w];
-        const float4 xq = Xs[w];
-        const float ex = xq.x - Xv[s][0], ey = xq.y - Xv[s][1], ez = xq.z - Xv[s][2];
+        const int4 xq = Xs[w];
+        // reference vector: exact integer difference, one rounding to FP32
+        const float ex = (float)(xq.x - Xv[s][0]) * xq_inv;
+        const float ey = (float)(xq.y - Xv[s][1]) * xq_inv;
+        const float ez = (float)(xq.z - Xv[s][2]) * xq_inv;
         const float gx = (q.x - uh[s][0]) - ul[s][0];
         const float gy = (q.y - uh[s][1]) - ul[s][1];
         const float gz = (q.z - uh[s][2]) - ul[s][2];
@@ -175,10 +181,11 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     const int32_t *fb = a.fib + 2 * R.fib_off;
     for (int e = tid; e < M; e += B) {
       const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
-      const float4 p1 = P[n1], p2 = P[n2], X1 = Xs[n1], X2 = Xs[n2];
-      const float x1 = (X2.x - X1.x) + (p2.x - p1.x);
-      const float x2 = (X2.y - X1.y) + (p2.y - p1.y);
-      const float x3 = (X2.z - X1.z) + (p2.z - p1.z);
+      const float4 p1 = P[n1], p2 = P[n2];
+      const int4 X1 = Xs[n1], X2 = Xs[n2];
+      const float x1 = (float)(X2.x - X1.x) * xq_inv + (p2.x - p1.x);
+      const float x2 = (float)(X2.y - X1.y) * xq_inv + (p2.y - p1.y);
+      const float x3 = (float)(X2.z - X1.z) * xq_inv + (p2.z - p1.z);
       lmin = fminf(lmin, x1 * x1 + x2 * x2 + x3 * x3);
     }
     return (double)lmin;
@@ -314,7 +321,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
       xb[d] = a.X[3 * gr + d] + ub[d];
     }
-    const float4 hb = Ph[r], Xb = Xs[r];
+    const float4 hb = Ph[r];
+    const int4 Xb = Xs[r];
     const float4 lb = kDbuf ? Pl[r] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int k = r / kWave;
     const int base = a.slice_base[R.slice_off + k] + (r & (kWave - 1));
@@ -325,9 +333,12 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const int32_t wv = gw[idx];
       const float il = gil[idx];
       const int w = wv & 0xFFFFFF;
-      const float4 qh = Ph[w], xq = Xs[w];
+      const float4 qh = Ph[w];
+      const int4 xq = Xs[w];
       const float4 ql = kDbuf ? Pl[w] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      const float ex = xq.x - Xb.x, ey = xq.y - Xb.y, ez = xq.z - Xb.z;
+      const float ex = (float)(xq.x - Xb.x) * xq_inv;
+      const float ey = (float)(xq.y - Xb.y) * xq_inv;
+      const float ez = (float)(xq.z - Xb.z) * xq_inv;
       const float gx = (qh.x - hb.x) + (ql.x - lb.x);
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
