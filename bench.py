@@ -78,7 +78,7 @@ def cpu_baseline(nets_fn, F, n_steps, target_s=12.0):
     t = time.perf_counter()
     o1.run(F[:1], update_coords=False)
     t1 = time.perf_counter() - t
-    n = int(max(threads, min(512, round(target_s / max(t1, 1e-3) * threads))))
+    n = int(max(threads, min(4096, round(target_s / max(t1, 1e-3) * threads))))
     n = max(threads, (n // threads) * threads)
     o = Oracle(nets_fn(n), p)
     t = time.perf_counter()
