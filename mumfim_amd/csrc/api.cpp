@@ -186,6 +186,7 @@ struct mf_batch {
   // plan: per RVE a variant index per precision
   std::vector<int> plan[2];
   std::vector<size_t> lds[2];
+  int no_reg_variants = 0;  // planner restricted to the LDS/global-list kernels
   // one-solve-per-RVE tables kept on the device: RVE ids, launch order per
   // precision, flags per (update) and the broadcast/gathered F buffer
   DevBuf<int32_t> id_rve, id_order[2], id_flags[2];
@@ -215,15 +216,22 @@ int make_plan(mf_batch *b, int prec) {
     const PackedRVE &p = b->host[r];
     int best = -1;
     long best_key = 0;
+    const bool lin = p.reactions.size() == 1 && p.reactions[0].type == MF_REACTION_LINEAR;
     for (int i = 0; i < nv; ++i) {
       const Variant &v = vs[i];
       if (v.prec != prec) continue;
       if (b->p.block_size && v.block != b->p.block_size) continue;
       if ((long)v.block * v.npt < p.Nf) continue;
+      const bool regs = (v.flags & kEllInRegs) != 0;
+      if (regs && p.max_free_deg > kRegDegree) continue;
+      if ((v.flags & kLinearOnly) && !lin) continue;
+      if (regs && b->no_reg_variants) continue;
       size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
       if (L > kMaxLds) continue;
-      // smaller thread count first, then fewer slots, then richer flags
-      long key = (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
+      // register-resident lists first, then smaller thread count, fewer
+      // slots, richer flags
+      long key = (regs ? 0L : 1000000000L) + (long)v.block * 1000000 +
+                 (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
         best = i;
         best_key = key;

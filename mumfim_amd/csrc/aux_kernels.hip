@@ -143,7 +143,16 @@ __global__ __launch_bounds__(B) void orientation_kernel(
 }  // namespace kern
 
 #define MF_VARIANT_ENTRY(P, B, N, F) {P, B, N, F},
-static const Variant kVariants[] = {MF_VARIANT_LIST(MF_VARIANT_ENTRY)};
+static const Variant kVariants[] = {
+    MF_VARIANT_LIST(MF_VARIANT_ENTRY)
+    // FP32c with incidence lists in registers (one free node per thread)
+    {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf},
+};
 
 const Variant *variants(int *count) {
   *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
@@ -156,6 +165,8 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
   size_t b = 0;
   if (v.prec == MF_PREC_FP64) {
     b += sizeof(double) * 3 * Np * ((v.flags & kDoubleBuf) ? 2 : 1);
+  } else if (v.flags & kEllInRegs) {
+    b += 16 * Np * 2;  // displacement buffers only
   } else {
     // displacement buffers + reference coordinates
     b += 16 * Np * ((v.flags & kDoubleBuf) ? 3 : 2);

@@ -79,6 +79,7 @@ struct PackedRVE {
   std::vector<int32_t> ell_w;
   std::vector<double> ell_l0;
   int ell_free_entries = 0;            // entries of slices that hold free nodes
+  int max_free_deg = 0;                // largest degree of a free node
   std::vector<int32_t> fib;            // 2M kernel ranks, input fiber order
   std::vector<double> l0;              // M
   std::vector<int32_t> rid;            // M (after the reaction policy)

@@ -42,9 +42,13 @@ struct SolveArgs {
 // A kernel variant: threads per solve, free-node slots per thread and the
 // placement flags.
 enum : int32_t {
-  kEllInLds = 1,   // free-node incidence lists staged in LDS
-  kDoubleBuf = 2,  // positions double-buffered in LDS (one barrier per step)
+  kEllInLds = 1,    // free-node incidence lists staged in LDS
+  kDoubleBuf = 2,   // positions double-buffered in LDS (one barrier per step)
+  kEllInRegs = 4,   // incidence lists in registers, one free node per thread
+  kLinearOnly = 8,  // every fiber of the RVE uses one linear reaction
 };
+// Largest node degree the register-resident variants hold.
+constexpr int kRegDegree = 12;
 
 struct Variant {
   int32_t prec;    // MF_PREC_*

@@ -390,6 +390,348 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   }
 }
 
+
+// ============================================================================
+// FP32 + compensated kernel, incidence lists in registers.  One free node per
+// thread; the thread keeps, for each of its (at most kRegDeg) fibers, the LDS
+// byte offset of the other end, the exactly rounded FP32 reference vector
+// d0 = X_w - X_v and either EA/(2 l0^3) (linear-only RVEs, t = that * s) or
+// 1/l0.  Entries past the node's degree point at the node itself with zero
+// coefficients, so every lane runs the wave-uniform trip count without masks.
+// A step reads one neighbour row (FP32 displacement hi) per fiber from LDS.
+// ============================================================================
+constexpr int kRegDeg = 12;
+
+template <int B, bool LIN>
+__global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int64_t nb = R.node_off;
+
+  size_t off = 0;
+  float4 *pos = reinterpret_cast<float4 *>(smem);  // [2][Np] displacement hi
+  off += sizeof(float4) * 2 * Np;
+  double *red = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  off += sizeof(double) * 6 * Nbp;
+  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const float *gil = a.ell_il + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  const double *X = a.X + 3 * nb;
+  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  const double EA_half = 0.5 * a.reacts[R.react_off].E * a.reacts[R.react_off].A;
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  auto init_node = [&](int r, double (&u)[3]) {
+    const int64_t g = nb + r;
+    double xp[3], up[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+    double inc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[k] = ramp ? up[k] : inc[k] + up[k];
+    const float4 p = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
+    pos[r] = p;
+    pos[Np + r] = p;
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = u[k];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  };
+
+  // own free node (rank tid) and its incidences
+  const int r = tid;
+  const bool own = r < Nf;
+  float uh[3], ul[3], vr[3] = {0, 0, 0}, ar[3] = {0, 0, 0}, imr = 0.0f;
+  int offs[kRegDeg];
+  float cf[kRegDeg], d0x[kRegDeg], d0y[kRegDeg], d0z[kRegDeg];
+  int D = 0;
+  {
+    double u[3] = {0, 0, 0};
+    if (r < N) init_node(r, u);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      uh[d] = (float)u[d];
+      ul[d] = (float)(u[d] - (double)uh[d]);
+    }
+    int dg = 0, base = 0;
+    if (own) {
+      imr = (float)(1.0 / a.mass[nb + r]);
+      dg = a.deg[nb + r];
+      const int k = r / kWave;
+      D = a.slice_D[R.slice_off + k];
+      base = a.slice_base[R.slice_off + k] + lane;
+    }
+    const int rr = r < Np ? r : 0;
+#pragma unroll
+    for (int j = 0; j < kRegDeg; ++j) {
+      offs[j] = rr * (int)sizeof(float4);
+      cf[j] = 0.0f;
+      d0x[j] = d0y[j] = d0z[j] = 0.0f;
+      if (j < dg) {
+        const int idx = base + j * kWave;
+        const int32_t wv = gw[idx];
+        const int w = wv & 0xFFFFFF;
+        offs[j] = w * (int)sizeof(float4) | (LIN ? 0 : (wv & 0xFF000000));
+        const double l0 = gl0[idx];
+        cf[j] = LIN ? (float)(EA_half / (l0 * l0 * l0)) : gil[idx];
+        d0x[j] = (float)(X[3 * w] - X[3 * r]);
+        d0y[j] = (float)(X[3 * w + 1] - X[3 * r + 1]);
+        d0z[j] = (float)(X[3 * w + 2] - X[3 * r + 2]);
+      }
+    }
+  }
+  D = __builtin_amdgcn_readfirstlane(D);
+  for (int rb = B + tid; rb < N; rb += B) {
+    double u[3];
+    init_node(rb, u);
+  }
+  __syncthreads();
+
+  const float inv_c = (float)(1.0 / R.c_sound);
+  const float eta = (float)a.eta;
+  int cur = 0;
+  // internal force of the own node from the displacements in buffer `buf`
+  auto node_force = [&](int buf, float (&f)[3]) {
+    const unsigned char *P = reinterpret_cast<const unsigned char *>(pos + buf * Np);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kRegDeg; ++j) {
+      if (j < D) {
+        const float4 q = *reinterpret_cast<const float4 *>(P + (offs[j] & 0xFFFFFF));
+        const float gx = (q.x - uh[0]) - ul[0];
+        const float gy = (q.y - uh[1]) - ul[1];
+        const float gz = (q.z - uh[2]) - ul[2];
+        const float dx = d0x[j] + gx, dy = d0y[j] + gy, dz = d0z[j] + gz;
+        // l^2 - l0^2 = (d0 + d) . delta
+        const float sl = (d0x[j] + dx) * gx + (d0y[j] + dy) * gy + (d0z[j] + dz) * gz;
+        float t;
+        if (LIN) {
+          t = cf[j] * sl;
+        } else {
+          const float il = cf[j];
+          t = react32_t(rxs[(uint32_t)offs[j] >> 24], sl * (il * il), il);
+        }
+        fx -= dx * t;
+        fy -= dy * t;
+        fz -= dz * t;
+      }
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  // min fiber length^2 over all fibers (every dt refresh)
+  auto min_length2 = [&](int buf) {
+    const float4 *P = pos + buf * Np;
+    float lmin = 3.0e38f;
+    const int32_t *fb = a.fib + 2 * R.fib_off;
+    for (int e = tid; e < M; e += B) {
+      const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+      const float4 p1 = P[n1], p2 = P[n2];
+      const float x1 = (float)(X[3 * n2] - X[3 * n1]) + (p2.x - p1.x);
+      const float x2 = (float)(X[3 * n2 + 1] - X[3 * n1 + 1]) + (p2.y - p1.y);
+      const float x3 = (float)(X[3 * n2 + 2] - X[3 * n1 + 2]) + (p2.z - p1.z);
+      lmin = fminf(lmin, x1 * x1 + x2 * x2 + x3 * x3);
+    }
+    return (double)lmin;
+  };
+
+  float fint[3] = {0, 0, 0};
+  double resid_local = 0.0;
+  if (own) {
+    node_force(cur, fint);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
+  }
+  double lmin2 = min_length2(cur);
+  block_min_sum<B>(lmin2, resid_local, red);
+  float dt_crit = sqrtf((float)lmin2) * inv_c;
+
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  const float crit_half = (float)(a.crit * 0.5);
+  const double inv_steps = 1.0 / (double)a.n_steps;
+  for (;;) {
+    const float h = crit_half * dt_crit;
+    const float h2 = 2.0f * h;
+    const int nxt = cur ^ 1;
+    float4 *Pn = pos + nxt * Np;
+    if (own) {
+      float pc[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float vl = vr[d] + h * ar[d];
+        vr[d] = vl;
+        const float y = ul[d] + h2 * vl;
+        float sh, e;
+        two_sum(uh[d], y, sh, e);
+        uh[d] = sh;
+        ul[d] = e;
+        pc[d] = sh;
+      }
+      Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
+    }
+    if (ramp) {
+      const double alpha = (double)(step + 1) * inv_steps;
+      for (int b = tid; b < Nb; b += B)
+        Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
+                                 (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
+                                 (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
+                                 0.0f);
+    }
+    __syncthreads();
+    cur = nxt;
+    const bool refresh = (step % freq) == 0;
+    double rsum = 0.0;
+    if (own) {
+      node_force(cur, fint);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
+      if (refresh)
+        rsum = (double)(fint[0] * fint[0] + fint[1] * fint[1] + fint[2] * fint[2]);
+    }
+    if (refresh) {
+      double lm = min_length2(cur);
+      block_min_sum<B>(lm, rsum, red);
+      dt_crit = sqrtf((float)lm) * inv_c;
+      residual = sqrt(rsum);
+    }
+    if (own) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) vr[d] += h * ar[d];
+    }
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // finalize: low parts into the spare buffer, boundary gathers in FP32 with
+  // exactly rounded reference vectors, stress in FP64
+  float4 *Ph = pos + cur * Np;
+  float4 *Pl = pos + (cur ^ 1) * Np;
+  __syncthreads();
+  if (own) Pl[r] = make_float4(ul[0], ul[1], ul[2], 0.0f);
+  const double alpha_end = ramp ? 1.0 : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    double ub[3];
+    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+    Ph[Nf + b] = hb;
+    Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
+                             (float)(ub[2] - (double)hb.z), 0.0f);
+  }
+  __syncthreads();
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const int flags = a.solve_flags[sid];
+  for (int b = tid; b < Nb; b += B) {
+    const int rb = Nf + b;
+    const int64_t gr = nb + rb;
+    double ub[3], xb[3];
+    for (int d = 0; d < 3; ++d) {
+      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+      xb[d] = X[3 * rb + d] + ub[d];
+    }
+    const float4 hb = Ph[rb], lb = Pl[rb];
+    const int k = rb / kWave;
+    const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
+    const int dg = a.deg[gr];
+    float f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const int w = wv & 0xFFFFFF;
+      const float4 qh = Ph[w], ql = Pl[w];
+      const float ex = (float)(X[3 * w] - X[3 * rb]);
+      const float ey = (float)(X[3 * w + 1] - X[3 * rb + 1]);
+      const float ez = (float)(X[3 * w + 2] - X[3 * rb + 2]);
+      const float gx = (qh.x - hb.x) + (ql.x - lb.x);
+      const float gy = (qh.y - hb.y) + (ql.y - lb.y);
+      const float gz = (qh.z - hb.z) + (ql.z - lb.z);
+      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      float t;
+      if (LIN) {
+        const double l0 = gl0[idx];
+        t = (float)(EA_half / (l0 * l0 * l0)) * sl;
+      } else {
+        const float il = gil[idx];
+        t = react32_t(rxs[(uint32_t)wv >> 24], sl * (il * il), il);
+      }
+      f1 -= dx * t;
+      f2 -= dy * t;
+      f3 -= dz * t;
+    }
+    const double g1 = f1, g2 = f2, g3 = f3;
+    st[0] += xb[0] * g1;
+    st[1] += xb[1] * g2;
+    st[2] += xb[2] * g3;
+    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
+    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
+    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
+  }
+  if (own) {
+    const int64_t g = nb + r;
+    for (int d = 0; d < 3; ++d) {
+      const double u = (double)uh[d] + (double)ul[d];
+      if (flags & kWriteX) a.x_out[3 * g + d] = X[3 * r + d] + u;
+      if (flags & kWriteU) a.u_out[3 * g + d] = u;
+    }
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !isfinite(v);
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
 }  // namespace kern
 
 hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
@@ -407,6 +749,18 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
+#define MF_REG_LAUNCH(B)                                                       \
+  if (v.npt == 1 && v.block == B && (v.flags & kEllInRegs)) {                  \
+    if (v.flags & kLinearOnly)                                                 \
+      hipLaunchKernelGGL((kern::solve32r<B, true>), dim3(n_blocks), dim3(B), lds, s, a); \
+    else                                                                       \
+      hipLaunchKernelGGL((kern::solve32r<B, false>), dim3(n_blocks), dim3(B), lds, s, a); \
+    return hipGetLastError();                                                  \
+  }
+  MF_REG_LAUNCH(256)
+  MF_REG_LAUNCH(512)
+  MF_REG_LAUNCH(1024)
+#undef MF_REG_LAUNCH
   return hipErrorInvalidValue;
 }
 
