@@ -221,9 +221,9 @@ int make_plan(mf_batch *b, int prec) {
       const Variant &v = vs[i];
       if (v.prec != prec) continue;
       if (b->p.block_size && v.block != b->p.block_size) continue;
-      if ((long)v.block * v.npt < p.Nf) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
-      if (regs && p.max_free_deg > kRegDegree) continue;
+      if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) continue;
+      if (regs && p.max_free_deg > 2 * kLaneDegree) continue;
       if ((v.flags & kLinearOnly) && !lin) continue;
       if (regs && b->no_reg_variants) continue;
       size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
@@ -271,6 +271,7 @@ int upload_batch(mf_batch *b) {
     d.slice_off = slice_off;
     d.n_slices = (int)p.slice_base.size();
     d.ell_free_entries = p.ell_free_entries;
+    d.n_split = p.n_split;
     d.react_off = (int)rx.size();
     d.n_react = (int)p.reactions.size();
     d.c_sound = p.c_sound;

@@ -61,6 +61,9 @@ int load_network_file(const std::string &path, Network &out);
 // Largest reaction table and ELL degree the kernels accept.
 constexpr int kMaxReactions = 8;
 constexpr int kSlice = 64;  // nodes per ELL slice = one wavefront
+// Fibers per lane in the register-resident FP32 kernel; nodes of higher
+// degree (up to twice this) take two lanes.
+constexpr int kLaneDegree = 6;
 
 // One RVE packed in kernel order: free nodes sorted by degree (descending,
 // stable), then boundary nodes in boundary-list order.
@@ -80,6 +83,7 @@ struct PackedRVE {
   std::vector<double> ell_l0;
   int ell_free_entries = 0;            // entries of slices that hold free nodes
   int max_free_deg = 0;                // largest degree of a free node
+  int n_split = 0;                     // free nodes with degree > kLaneDegree
   std::vector<int32_t> fib;            // 2M kernel ranks, input fiber order
   std::vector<double> l0;              // M
   std::vector<int32_t> rid;            // M (after the reaction policy)
@@ -109,7 +113,9 @@ struct RveDev {
   // rint(X * xq_scale) fits int32; xq_scale is a power of two.
   double xq_scale;
   float xq_inv;
-  int32_t pad2_;
+  // Free nodes of degree above the per-lane fiber count of the
+  // register-resident kernel; they are the first n_split free ranks.
+  int32_t n_split;
 };
 
 // Reaction constants as the kernels consume them.

@@ -47,8 +47,7 @@ enum : int32_t {
   kEllInRegs = 4,   // incidence lists in registers, one free node per thread
   kLinearOnly = 8,  // every fiber of the RVE uses one linear reaction
 };
-// Largest node degree the register-resident variants hold.
-constexpr int kRegDegree = 12;
+
 
 struct Variant {
   int32_t prec;    // MF_PREC_*

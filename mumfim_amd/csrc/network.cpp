@@ -370,7 +370,11 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
     p.mass[p.fib[2 * e + 1]] += c;
   }
   p.max_free_deg = 0;
-  for (int r = 0; r < p.Nf; ++r) p.max_free_deg = std::max(p.max_free_deg, p.deg[r]);
+  p.n_split = 0;
+  for (int r = 0; r < p.Nf; ++r) {
+    p.max_free_deg = std::max(p.max_free_deg, p.deg[r]);
+    if (p.deg[r] > kLaneDegree) p.n_split++;
+  }
   // CSR in kernel order, entries in increasing fiber index
   std::vector<int> row(N + 1, 0);
   for (int r = 0; r < N; ++r) row[r + 1] = row[r] + p.deg[r];

@@ -1,5 +1,7 @@
 // FP32 + compensated explicit solve: FP32 stretch, force and update
 // arithmetic, (hi, lo) FP32 positions advanced with TwoSum, FP64 stress.
+#include <type_traits>
+
 #include "kernels_common.hpp"
 
 namespace mf {
@@ -392,15 +394,18 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 
 
 // ============================================================================
-// FP32 + compensated kernel, incidence lists in registers.  One free node per
-// thread; the thread keeps, for each of its (at most kRegDeg) fibers, the LDS
-// byte offset of the other end, the exactly rounded FP32 reference vector
-// d0 = X_w - X_v and either EA/(2 l0^3) (linear-only RVEs, t = that * s) or
-// 1/l0.  Entries past the node's degree point at the node itself with zero
-// coefficients, so every lane runs the wave-uniform trip count without masks.
-// A step reads one neighbour row (FP32 displacement hi) per fiber from LDS.
+// FP32 + compensated kernel with per-lane fiber lists in registers.
+// Free nodes (degree-sorted, descending) map to lanes; a node of degree above
+// kLaneDeg takes two adjacent lanes (even/odd) that each gather half of its
+// fibers and exchange the partial forces with one DPP quad swap.  Every lane
+// therefore holds at most kLaneDeg fibers: the LDS byte offset of the other
+// end, the exactly rounded FP32 reference vector d0 = X_w - X_v and either
+// EA/(2 l0^3) (linear-only RVEs, t = that * s) or 1/l0.  Entries past a
+// lane's count address the node's own row with zero constants and contribute
+// exactly zero, so each wave runs its uniform trip count without masks.  A
+// step reads one neighbour row (FP32 displacement hi) per fiber from LDS.
 // ============================================================================
-constexpr int kRegDeg = 12;
+constexpr int kLaneDeg = kLaneDegree;
 
 template <int B, bool LIN>
 __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
@@ -463,38 +468,61 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       }
     }
   };
+  for (int rb = tid; rb < N; rb += B) {
+    double u[3];
+    init_node(rb, u);
+  }
 
-  // own free node (rank tid) and its incidences
-  const int r = tid;
-  const bool own = r < Nf;
-  float uh[3], ul[3], vr[3] = {0, 0, 0}, ar[3] = {0, 0, 0}, imr = 0.0f;
-  int offs[kRegDeg];
-  float cf[kRegDeg], d0x[kRegDeg], d0y[kRegDeg], d0z[kRegDeg];
-  int D = 0;
+  // lane -> (node rank, fiber range): the first S free ranks have degree >
+  // kLaneDeg (degree-sorted order) and take lanes 2i, 2i+1.
+  const int S = R.n_split;
+  const bool split = tid < 2 * S;
+  const int r = split ? tid >> 1 : tid - S;
+  const bool own = r < Nf;                   // lane computes a node
+  const bool writer = own && (!split || !(tid & 1));  // lane publishes it
+  float uh[3] = {0, 0, 0}, ul[3] = {0, 0, 0}, vr[3] = {0, 0, 0}, ar[3] = {0, 0, 0};
+  float imr = 0.0f;
+  int offs[kLaneDeg];
+  float cf[kLaneDeg], d0x[kLaneDeg], d0y[kLaneDeg], d0z[kLaneDeg];
+  int cnt = 0;
   {
-    double u[3] = {0, 0, 0};
-    if (r < N) init_node(r, u);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      uh[d] = (float)u[d];
-      ul[d] = (float)(u[d] - (double)uh[d]);
-    }
-    int dg = 0, base = 0;
+    int j0 = 0, dg = 0, base = 0;
     if (own) {
-      imr = (float)(1.0 / a.mass[nb + r]);
-      dg = a.deg[nb + r];
+      const int64_t g = nb + r;
+      double u[3];
+      double xp[3], up[3];
+      for (int d = 0; d < 3; ++d) {
+        xp[d] = a.x_in[3 * g + d];
+        up[d] = a.u_in[3 * g + d];
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double inc = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+                           (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+                           (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+        u[k] = ramp ? up[k] : inc + up[k];
+        uh[k] = (float)u[k];
+        ul[k] = (float)(u[k] - (double)uh[k]);
+      }
+      imr = (float)(1.0 / a.mass[g]);
+      dg = a.deg[g];
       const int k = r / kWave;
-      D = a.slice_D[R.slice_off + k];
-      base = a.slice_base[R.slice_off + k] + lane;
+      base = a.slice_base[R.slice_off + k] + (r & (kWave - 1));
+      if (split) {
+        const int h = (dg + 1) >> 1;
+        j0 = (tid & 1) ? h : 0;
+        cnt = (tid & 1) ? dg - h : h;
+      } else {
+        cnt = dg;
+      }
     }
-    const int rr = r < Np ? r : 0;
+    const int self = (own ? r : 0) * (int)sizeof(float4);
 #pragma unroll
-    for (int j = 0; j < kRegDeg; ++j) {
-      offs[j] = rr * (int)sizeof(float4);
+    for (int j = 0; j < kLaneDeg; ++j) {
+      offs[j] = self;
       cf[j] = 0.0f;
       d0x[j] = d0y[j] = d0z[j] = 0.0f;
-      if (j < dg) {
-        const int idx = base + j * kWave;
+      if (j < cnt) {
+        const int idx = base + (j0 + j) * kWave;
         const int32_t wv = gw[idx];
         const int w = wv & 0xFFFFFF;
         offs[j] = w * (int)sizeof(float4) | (LIN ? 0 : (wv & 0xFF000000));
@@ -506,27 +534,33 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       }
     }
   }
+  // wave-uniform trip count = largest per-lane count in the wave
+  int D = cnt;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) D = max(D, __shfl_xor(D, o, kWave));
   D = __builtin_amdgcn_readfirstlane(D);
-  for (int rb = B + tid; rb < N; rb += B) {
-    double u[3];
-    init_node(rb, u);
-  }
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
   const float eta = (float)a.eta;
   int cur = 0;
-  // internal force of the own node from the displacements in buffer `buf`
+  // internal force of the lane's node (both halves for a split node).
+  // The wave-uniform trip count selects a straight-line body, so the
+  // neighbour-row loads issue ahead of the arithmetic.
   auto node_force = [&](int buf, float (&f)[3]) {
     const unsigned char *P = reinterpret_cast<const unsigned char *>(pos + buf * Np);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    auto body = [&](auto KC) {
+      constexpr int K = decltype(KC)::value;
+      float4 q[K];
 #pragma unroll
-    for (int j = 0; j < kRegDeg; ++j) {
-      if (j < D) {
-        const float4 q = *reinterpret_cast<const float4 *>(P + (offs[j] & 0xFFFFFF));
-        const float gx = (q.x - uh[0]) - ul[0];
-        const float gy = (q.y - uh[1]) - ul[1];
-        const float gz = (q.z - uh[2]) - ul[2];
+      for (int j = 0; j < K; ++j)
+        q[j] = *reinterpret_cast<const float4 *>(P + (LIN ? offs[j] : (offs[j] & 0xFFFFFF)));
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const float gx = (q[j].x - uh[0]) - ul[0];
+        const float gy = (q[j].y - uh[1]) - ul[1];
+        const float gz = (q[j].z - uh[2]) - ul[2];
         const float dx = d0x[j] + gx, dy = d0y[j] + gy, dz = d0z[j] + gz;
         // l^2 - l0^2 = (d0 + d) . delta
         const float sl = (d0x[j] + dx) * gx + (d0y[j] + dy) * gy + (d0z[j] + dz) * gz;
@@ -541,10 +575,25 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         fy -= dy * t;
         fz -= dz * t;
       }
+    };
+    switch (D) {
+#define MF_DEG_CASE(k) \
+  case k:              \
+    body(std::integral_constant<int, k>{}); \
+    break;
+      MF_DEG_CASE(1) MF_DEG_CASE(2) MF_DEG_CASE(3)
+      MF_DEG_CASE(4) MF_DEG_CASE(5) MF_DEG_CASE(6)
+#undef MF_DEG_CASE
+      default:
+        break;
     }
-    f[0] = fx;
-    f[1] = fy;
-    f[2] = fz;
+    // combine the halves of split nodes: lanes 2i and 2i+1 swap partials
+    const float px = __builtin_amdgcn_update_dpp(0.0f, fx, 0xB1, 0xF, 0xF, false);
+    const float py = __builtin_amdgcn_update_dpp(0.0f, fy, 0xB1, 0xF, 0xF, false);
+    const float pz = __builtin_amdgcn_update_dpp(0.0f, fz, 0xB1, 0xF, 0xF, false);
+    f[0] = split ? fx + px : fx;
+    f[1] = split ? fy + py : fy;
+    f[2] = split ? fz + pz : fz;
   };
   // min fiber length^2 over all fibers (every dt refresh)
   auto min_length2 = [&](int buf) {
@@ -597,7 +646,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         ul[d] = e;
         pc[d] = sh;
       }
-      Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
+      if (writer) Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
     }
     if (ramp) {
       const double alpha = (double)(step + 1) * inv_steps;
@@ -615,7 +664,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       node_force(cur, fint);
 #pragma unroll
       for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
-      if (refresh)
+      if (refresh && writer)
         rsum = (double)(fint[0] * fint[0] + fint[1] * fint[1] + fint[2] * fint[2]);
     }
     if (refresh) {
@@ -646,7 +695,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   float4 *Ph = pos + cur * Np;
   float4 *Pl = pos + (cur ^ 1) * Np;
   __syncthreads();
-  if (own) Pl[r] = make_float4(ul[0], ul[1], ul[2], 0.0f);
+  if (writer) Pl[r] = make_float4(ul[0], ul[1], ul[2], 0.0f);
   const double alpha_end = ramp ? 1.0 : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
@@ -709,7 +758,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     if (flags & kWriteU)
       for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
   }
-  if (own) {
+  if (writer) {
     const int64_t g = nb + r;
     for (int d = 0; d < 3; ++d) {
       const double u = (double)uh[d] + (double)ul[d];
