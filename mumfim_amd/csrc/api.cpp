@@ -166,6 +166,10 @@ struct mf_batch {
   int64_t total_nodes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // extra streams so launch groups of different kernel variants overlap
+  static constexpr int kAux = 3;
+  hipStream_t aux[kAux] = {nullptr, nullptr, nullptr};
+  hipEvent_t aux_done[kAux] = {nullptr, nullptr, nullptr};
   float last_ms = 0;
   int last_launches = 0;
   DevBuf<RveDev> rves;
@@ -198,6 +202,10 @@ struct mf_batch {
   ~mf_batch() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (int i = 0; i < kAux; ++i) {
+      if (aux_done[i]) (void)hipEventDestroy(aux_done[i]);
+      if (aux[i]) (void)hipStreamDestroy(aux[i]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -332,6 +340,10 @@ int upload_batch(mf_batch *b) {
 }
 
 int build_identity_tables(mf_batch *b);
+void fill_args(mf_batch *b, SolveArgs &a);
+int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
+                  const std::vector<std::pair<int, int>> &groups,
+                  const std::vector<size_t> &lds);
 
 int create_from_networks(std::vector<Network> &nets, const mf_solver_params *params,
                          mf_batch **out) {
@@ -376,6 +388,10 @@ int create_from_networks(std::vector<Network> &nets, const mf_solver_params *par
   MF_HIP(hipStreamCreate(&b->stream));
   MF_HIP(hipEventCreate(&b->ev0));
   MF_HIP(hipEventCreate(&b->ev1));
+  for (int i = 0; i < mf_batch::kAux; ++i) {
+    MF_HIP(hipStreamCreate(&b->aux[i]));
+    MF_HIP(hipEventCreateWithFlags(&b->aux_done[i], hipEventDisableTiming));
+  }
   int e = upload_batch(b.get());
   if (e != MF_OK) return e;
   e = build_identity_tables(b.get());
@@ -406,8 +422,6 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   MF_HIP(b->steps.ensure(n));
   // group by variant, largest RVEs first inside a group
   const int prec = S.prec;
-  int nv = 0;
-  const Variant *vs = variants(&nv);
   std::vector<int32_t> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
@@ -454,8 +468,8 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.eta = b->p.visc_damp_coeff;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
-  MF_HIP(hipEventRecord(b->ev0, s));
-  int launches = 0;
+  std::vector<std::pair<int, int>> groups;
+  std::vector<size_t> glds;
   for (int i = 0; i < n;) {
     int v = b->plan[prec][S.rve[order[i]]];
     int j = i;
@@ -464,18 +478,11 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
       L = std::max(L, b->lds[prec][S.rve[order[j]]]);
       ++j;
     }
-    a.order = b->order.p + i;
-    hipError_t e = launch_solve(vs[v], a, j - i, L, s);
-    if (e != hipSuccess) {
-      set_error(std::string("solve launch: ") + hipGetErrorString(e));
-      return MF_ERR_HIP;
-    }
-    ++launches;
+    groups.push_back({v, j - i});
+    glds.push_back(L);
     i = j;
   }
-  MF_HIP(hipEventRecord(b->ev1, s));
-  b->last_launches = launches;
-  return MF_OK;
+  return launch_groups(b, a, b->order.p, groups, glds);
 }
 
 void fill_args(mf_batch *b, SolveArgs &a) {
@@ -532,7 +539,7 @@ int build_identity_tables(mf_batch *b) {
       i = j;
     }
     MF_HIP(b->id_order[prec].upload(order, s));
-    // flags[prec] reused as: [0..R) update=false, [R..2R) update=true
+    // flags: [0..R) update=false, [R..2R) update=true
     std::vector<int32_t> fl(2 * (size_t)R);
     const bool ramp = b->p.mode == MF_MODE_RAMP;
     for (int r = 0; r < R; ++r) {
@@ -542,6 +549,44 @@ int build_identity_tables(mf_batch *b) {
     MF_HIP(b->id_flags[prec].upload(fl, s));
   }
   MF_HIP(hipStreamSynchronize(s));
+  return MF_OK;
+}
+
+// Launches groups (variant, solve count, LDS bytes) of consecutive entries
+// of `order`; group 0 runs on the batch stream, the others on the auxiliary
+// streams, all after ev0 and joined back before ev1.
+int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
+                  const std::vector<std::pair<int, int>> &groups,
+                  const std::vector<size_t> &lds) {
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  hipStream_t s = b->stream;
+  MF_HIP(hipEventRecord(b->ev0, s));
+  bool used[mf_batch::kAux] = {false, false, false};
+  int off = 0;
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    hipStream_t st = s;
+    if (gi > 0) {
+      const int k = (int)((gi - 1) % mf_batch::kAux);
+      st = b->aux[k];
+      if (!used[k]) MF_HIP(hipStreamWaitEvent(st, b->ev0, 0));
+      used[k] = true;
+    }
+    a.order = order + off;
+    hipError_t e = launch_solve(vs[groups[gi].first], a, groups[gi].second, lds[gi], st);
+    if (e != hipSuccess) {
+      set_error(std::string("solve launch: ") + hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+    off += groups[gi].second;
+  }
+  for (int k = 0; k < mf_batch::kAux; ++k)
+    if (used[k]) {
+      MF_HIP(hipEventRecord(b->aux_done[k], b->aux[k]));
+      MF_HIP(hipStreamWaitEvent(s, b->aux_done[k], 0));
+    }
+  MF_HIP(hipEventRecord(b->ev1, s));
+  b->last_launches = (int)groups.size();
   return MF_OK;
 }
 
@@ -604,23 +649,11 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
   a.solve_flags = b->id_flags[prec].p + (update ? R : 0);
   a.sigma = sigma_dev ? sigma_dev : b->sigma.p;
   a.status = status_dev ? status_dev : b->status.p;
-  int nv = 0;
-  const Variant *vs = variants(&nv);
-  MF_HIP(hipEventRecord(b->ev0, s));
-  int off = 0, launches = 0;
-  for (auto &g : b->id_groups[prec]) {
-    a.order = b->id_order[prec].p + off;
-    hipError_t e = launch_solve(vs[g.first], a, g.second,
-                                b->id_group_lds[prec][launches], s);
-    if (e != hipSuccess) {
-      set_error(std::string("solve launch: ") + hipGetErrorString(e));
-      return MF_ERR_HIP;
-    }
-    off += g.second;
-    ++launches;
+  {
+    int e = launch_groups(b, a, b->id_order[prec].p, b->id_groups[prec],
+                          b->id_group_lds[prec]);
+    if (e != MF_OK) return e;
   }
-  MF_HIP(hipEventRecord(b->ev1, s));
-  b->last_launches = launches;
   if (update) {
     MF_HIP(hipMemcpyAsync(b->cur_stress.p, a.sigma, sizeof(double) * 6 * R,
                           hipMemcpyDeviceToDevice, s));

@@ -411,7 +411,7 @@ template <int B, bool LIN>
 __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int tid = threadIdx.x;
   const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   const int rv = a.solve_rve[sid];
   const RveDev R = a.rves[rv];
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   // internal force of the lane's node (both halves for a split node).
   // The wave-uniform trip count selects a straight-line body, so the
   // neighbour-row loads issue ahead of the arithmetic.
-  auto node_force = [&](int buf, float (&f)[3]) {
+  auto node_force = [&](int buf, float (&f)[3], bool any_split) {
     const unsigned char *P = reinterpret_cast<const unsigned char *>(pos + buf * Np);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](auto KC) {
@@ -588,12 +588,19 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         break;
     }
     // combine the halves of split nodes: lanes 2i and 2i+1 swap partials
-    const float px = __builtin_amdgcn_update_dpp(0.0f, fx, 0xB1, 0xF, 0xF, false);
-    const float py = __builtin_amdgcn_update_dpp(0.0f, fy, 0xB1, 0xF, 0xF, false);
-    const float pz = __builtin_amdgcn_update_dpp(0.0f, fz, 0xB1, 0xF, 0xF, false);
-    f[0] = split ? fx + px : fx;
-    f[1] = split ? fy + py : fy;
-    f[2] = split ? fz + pz : fz;
+    if (any_split) {
+      const float px = __builtin_amdgcn_update_dpp(0.0f, fx, 0xB1, 0xF, 0xF, false);
+      const float py = __builtin_amdgcn_update_dpp(0.0f, fy, 0xB1, 0xF, 0xF, false);
+      const float pz = __builtin_amdgcn_update_dpp(0.0f, fz, 0xB1, 0xF, 0xF, false);
+      if (split) {
+        fx += px;
+        fy += py;
+        fz += pz;
+      }
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
   };
   // min fiber length^2 over all fibers (every dt refresh)
   auto min_length2 = [&](int buf) {
@@ -614,7 +621,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   float fint[3] = {0, 0, 0};
   double resid_local = 0.0;
   if (own) {
-    node_force(cur, fint);
+    node_force(cur, fint, true);
 #pragma unroll
     for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
   }
@@ -628,22 +635,31 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   const int freq = a.freq;
   const float crit_half = (float)(a.crit * 0.5);
   const double inv_steps = 1.0 / (double)a.n_steps;
+  // The closing half kick of one step and the opening half kick of the next
+  // use the same acceleration: v_{n+1/2} = v_{n-1/2} + (h_{n-1} + h_n) a_n,
+  // with v_{-1/2} = 0 and h_{-1} = 0 (velocity starts at rest).
+  float h_prev = 0.0f;
+  const bool wave_split = __builtin_amdgcn_readfirstlane(tid & ~(kWave - 1)) < 2 * S;
   for (;;) {
     const float h = crit_half * dt_crit;
+    const float hk = h_prev + h;
     const float h2 = 2.0f * h;
+    h_prev = h;
     const int nxt = cur ^ 1;
     float4 *Pn = pos + nxt * Np;
     if (own) {
       float pc[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const float vl = vr[d] + h * ar[d];
+        const float vl = vr[d] + hk * ar[d];
         vr[d] = vl;
+        // u (hi, lo) += 2h v with Fast2Sum: exact when |u_hi| >= |increment|;
+        // otherwise u is within a few increments of zero and the
+        // uncompensated rounding is below eps * |increment|.
         const float y = ul[d] + h2 * vl;
-        float sh, e;
-        two_sum(uh[d], y, sh, e);
+        const float sh = uh[d] + y;
+        ul[d] = y - (sh - uh[d]);
         uh[d] = sh;
-        ul[d] = e;
         pc[d] = sh;
       }
       if (writer) Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
@@ -661,7 +677,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     const bool refresh = (step % freq) == 0;
     double rsum = 0.0;
     if (own) {
-      node_force(cur, fint);
+      node_force(cur, fint, wave_split);
 #pragma unroll
       for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
       if (refresh && writer)
@@ -672,10 +688,6 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       block_min_sum<B>(lm, rsum, red);
       dt_crit = sqrtf((float)lm) * inv_c;
       residual = sqrt(rsum);
-    }
-    if (own) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) vr[d] += h * ar[d];
     }
     ++step;
     if (ramp) {
