@@ -25,6 +25,18 @@ METRIC = "RVE solves/s at ≤1e-4 rel stress err vs FP64; fraction of BW rooflin
 WORKLOAD = dict(n_rve=4096, n_nodes=500, k=4, explicit_steps=5000, F_amp=0.05)
 # MI355X LDS read bandwidth from the data sheet: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_SPEC = 256 * 128 * 2.4e9
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# bench (scaled by a plain copy kernel of known size), see profiles/
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(solver):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f).get(solver)
+        return None if t is None else int(t["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(msg):
@@ -165,7 +177,7 @@ def main():
             "peak": round(lds_meas / 1e9, 1),
             "unit": "GB/s",
             "frac": round(achieved / lds_meas, 4),
-            "traffic": None,
+            "traffic": measured_traffic(head),
             "peak_spec": round(LDS_SPEC / 1e9, 1),
             "frac_of_spec": round(achieved / LDS_SPEC, 4),
             "kernel_ms": round(kern_ms, 3),
