@@ -136,7 +136,7 @@ bool params_valid(const mf_solver_params &p) {
          p.fd_step < 0.25 && p.crit_time_scale_factor > 0 &&
          p.visc_damp_coeff >= 0 &&
          (p.block_size == 0 || p.block_size == 256 || p.block_size == 512 ||
-          p.block_size == 1024);
+          p.block_size == 640 || p.block_size == 768 || p.block_size == 1024);
 }
 
 template <class F>

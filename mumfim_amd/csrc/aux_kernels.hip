@@ -148,9 +148,13 @@ static const Variant kVariants[] = {
     // FP32c with incidence lists in registers (one free node per thread)
     {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 640, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 768, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 640, 1, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 768, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf},
 };
 

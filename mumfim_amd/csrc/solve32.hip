@@ -558,9 +558,11 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         q[j] = *reinterpret_cast<const float4 *>(P + (LIN ? offs[j] : (offs[j] & 0xFFFFFF)));
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        const float gx = (q[j].x - uh[0]) - ul[0];
-        const float gy = (q[j].y - uh[1]) - ul[1];
-        const float gz = (q[j].z - uh[2]) - ul[2];
+        // both ends as their FP32-rounded displacements (the low parts only
+        // keep the accumulation exact)
+        const float gx = q[j].x - uh[0];
+        const float gy = q[j].y - uh[1];
+        const float gz = q[j].z - uh[2];
         const float dx = d0x[j] + gx, dy = d0y[j] + gy, dz = d0z[j] + gz;
         // l^2 - l0^2 = (d0 + d) . delta
         const float sl = (d0x[j] + dx) * gx + (d0y[j] + dy) * gy + (d0z[j] + dz) * gz;
@@ -820,6 +822,8 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_REG_LAUNCH(256)
   MF_REG_LAUNCH(512)
+  MF_REG_LAUNCH(640)
+  MF_REG_LAUNCH(768)
   MF_REG_LAUNCH(1024)
 #undef MF_REG_LAUNCH
   return hipErrorInvalidValue;
