@@ -100,6 +100,105 @@ def cpu_baseline(nets_fn, F, n_steps, target_s=12.0):
     return n / dt, n, threads, dt, s
 
 
+def rel_err(a, b):
+    return np.abs(a - b).max(1) / np.abs(b).max(1)
+
+
+def bench_cfg3(args):
+    """16384 RVEs, N log-uniform in [200, 4000], half the fibers on the
+    nonlinear toe reaction, F = I + sym(U) with U ~ U(-0.1, 0.1), 10000 ramp
+    steps, FP32c in one batch (all kernel-variant groups launched together);
+    FP64 on a 256-RVE subset for the error."""
+    import mumfim_amd as mf
+    from oracle.oracle import Oracle
+    R = 16384 if args.n_rve == WORKLOAD["n_rve"] else args.n_rve
+    S = 10000 if args.explicit_steps == WORKLOAD["explicit_steps"] else args.explicit_steps
+    u = mf.uniform(3, 11, R)
+    sizes = np.rint(np.exp(np.log(200) + u * (np.log(4000) - np.log(200)))).astype(np.int32)
+    F = strain_F(3, R, 0.1)
+    gen = mf.default_gen_params(seed=100001, nonlinear_fraction=0.5)
+    base = dict(mode=mf.MF_MODE_RAMP, n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER)
+    t = time.perf_counter()
+    g = mf.BatchedFiberRVEAnalysisExplicit(
+        params=mf.default_params(precision=mf.MF_PREC_FP32C, **base),
+        generated=(gen, R), n_nodes=sizes)
+    log(f"[cfg3] batch built in {time.perf_counter() - t:.1f}s, plan groups {g.plan()[0]}")
+    F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
+    sig = mf.DeviceArray((R, 6))
+    wall, kms = time_solver(g, F_dev, sig, args.steps, args.warmup)
+    s32 = sig.numpy()
+    st = g.stats()
+    nsub = 256
+    sub = np.arange(0, R, R // nsub)[:nsub]
+    nets = [mf.generate_network(seed=100001 + int(r), n_nodes=int(sizes[r]),
+                                nonlinear_fraction=0.5) for r in sub]
+    g64 = mf.BatchedFiberRVEAnalysisExplicit(
+        nets, mf.default_params(precision=mf.MF_PREC_FP64, **base))
+    s64 = g64.run(F[sub], update_coords=False)
+    e = rel_err(s32[sub], s64)
+    res = {"config": "cfg3", "workload": f"{R} RVEs, nodes log-uniform [200,4000] "
+           f"(total {int(st['N'].sum())} nodes, {int(st['M'].sum())} fibers), 50% nonlinear "
+           f"fibers, F=I+sym(U) U~U(-0.1,0.1), {S} ramp steps",
+           "value": round(R * args.steps / wall, 3), "unit": "RVE solves/s", "dtype": "f32",
+           "ms_per_batch": round(1e3 * wall / args.steps, 3),
+           "kernel_ms": round(float(np.mean(kms)), 3), "launch_groups": g.plan()[0],
+           "rel_stress_err_vs_fp64_subset": {"n": nsub, "max": float(e.max()),
+                                              "median": float(np.median(e))}}
+    if not args.no_cpu:
+        o = Oracle(nets[:32], mf.default_params(precision=mf.MF_PREC_FP64, **base))
+        so = o.run(F[sub[:32]], update_coords=False)
+        res["fp64_gpu_vs_oracle_max"] = float(rel_err(s64[:32], so).max())
+    print(json.dumps(res), flush=True)
+
+
+def bench_cfg4(args):
+    """4096 RVEs of 1000-2000 nodes, mixed reactions; per RVE the base solve
+    plus 12 central perturbations (h = 1e-6) in one batched FP64 launch
+    (53248 solves), FD tangent in FP64; tangent error vs the oracle on a
+    sample."""
+    import mumfim_amd as mf
+    from oracle.oracle import Oracle
+    R = args.n_rve
+    S = args.explicit_steps
+    u = mf.uniform(4, 11, R)
+    sizes = (1000 + np.floor(u * 1001)).astype(np.int32)
+    F = strain_F(4, R, 0.05)
+    gen = mf.default_gen_params(seed=200001, nonlinear_fraction=0.5)
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S,
+                          reaction_policy=mf.MF_REACT_PER_FIBER,
+                          tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6)
+    g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R), n_nodes=sizes)
+    times = []
+    first = None
+    for k in range(max(1, args.warmup) + args.steps):
+        t = time.perf_counter()
+        sigma, Cm = g.run_with_tangent(F)  # commits: later batches start deformed
+        dt = time.perf_counter() - t
+        if first is None:
+            first = (sigma, Cm)
+        elif len(times) < args.steps:
+            times.append(dt)
+    sigma, Cm = first
+    wall = float(np.mean(times))
+    res = {"config": "cfg4", "workload": f"{R} RVEs, nodes U[1000,2000], 50% nonlinear fibers, "
+           f"base + 12 central perturbations (h=1e-6) per RVE = {13 * R} solves of {S} ramp "
+           f"steps, FP64 solves + FP64 differencing",
+           "value": round(13 * R / wall, 3), "unit": "RVE solves/s (incl. perturbed)",
+           "tangents_per_s": round(R / wall, 3), "ms_per_batch": round(1e3 * wall, 3),
+           "dtype": "f64"}
+    if not args.no_cpu:
+        n = 16
+        nets = [mf.generate_network(seed=200001 + r, n_nodes=int(sizes[r]),
+                                    nonlinear_fraction=0.5) for r in range(n)]
+        o = Oracle(nets, p)
+        so = o.run(F[:n])
+        Co = o.computeMaterialStiffness()
+        res["sigma_err_vs_oracle_max"] = float(rel_err(sigma[:n], so).max())
+        res["tangent_err_vs_oracle_max"] = float(
+            rel_err(Cm[:n].reshape(n, 36), Co.reshape(n, 36)).max())
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +211,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
+    ap.add_argument("--config", choices=["cfg2", "cfg3", "cfg4"], default="cfg2",
+                    help="cfg2 (default): 4096 x 500 nodes; cfg3: 16384 RVEs of 200-4000 "
+                         "nodes, mixed reactions, 10000 steps; cfg4: 4096 RVEs of "
+                         "1000-2000 nodes, base + 12 perturbed solves per RVE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -120,6 +223,10 @@ def main():
 
     import mumfim_amd as mf
 
+    if args.config == "cfg3":
+        return bench_cfg3(args)
+    if args.config == "cfg4":
+        return bench_cfg4(args)
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])
