@@ -238,30 +238,113 @@ int mf_load_network(const char *path, int32_t *n_nodes, int32_t *n_fibers,
                     int32_t *fiber_reaction, int32_t *boundary,
                     mf_reaction *reactions);
 
-/* ---- learned surrogate (BF16 MFMA MLP 10 -> 256 -> 256 -> 256 -> 6) ------ */
+/* ---- learned surrogate (no reference counterpart) -------------------------
+ * An ensemble of MLPs 10 -> 256 -> 256 -> 256 -> 6 with SiLU activations.
+ * Weights and activations are BF16 on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16, FP32 accumulate); biases, master weights and
+ * Adam state are FP32.  Inputs x (10 per sample): the strain
+ * e = [e_xx, e_yy, e_zz, g_yz, g_xz, g_xy] of sym(F) - I (engineering shear
+ * g = 2 e_ij), node count, fibers per unit volume, mean fiber length,
+ * nonlinear-fiber fraction.  Outputs y (6): Cauchy stress, Voigt order. */
+#define MF_MLP_IN 10
+#define MF_MLP_HIDDEN 256
+#define MF_MLP_OUT 6
+#define MF_MLP_MAX_MEMBERS 8
+#define MF_MLP_BATCH_ALIGN 256 /* training mini-batches are multiples    */
+
 typedef struct mf_mlp mf_mlp;
-int mf_mlp_create(uint64_t seed, mf_mlp **out);
+/* n_members independent networks (1..MF_MLP_MAX_MEMBERS), member k
+ * initialised from SplitMix64 stream (seed, 1000 + k). */
+int mf_mlp_create(int32_t n_members, uint64_t seed, mf_mlp **out);
 void mf_mlp_destroy(mf_mlp *m);
-int mf_mlp_num_params(const mf_mlp *m);
-/* Input/output normalisation: x_n = (x - mean)/std, y = y_n*std + mean. */
+/* Parameters per member, packed W1,b1,W2,b2,W3,b3,W4,b4 (W row-major
+ * out x in): 135942. */
+int32_t mf_mlp_num_params(void);
+int32_t mf_mlp_num_members(const mf_mlp *m);
+/* Normalisation: x_n = (x - in_mean) * (1/in_std), y = y_n * out_std +
+ * out_mean (FP32; a std <= 0 counts as 1). */
 int mf_mlp_set_normalization(mf_mlp *m, const float *in_mean,
                              const float *in_std, const float *out_mean,
                              const float *out_std);
-/* Master FP32 weights, packed W1,b1,W2,b2,W3,b3,W4,b4 (W row-major out x in). */
-int mf_mlp_get_weights(mf_mlp *m, float *w);
-int mf_mlp_set_weights(mf_mlp *m, const float *w);
-/* Inference on device buffers: X (n x 10, FP32), Y (n x 6, FP32), optional
- * dYdX (n x 6 x 10, FP32) from the hand-written input-Jacobian backward. */
-int mf_mlp_forward_device(mf_mlp *m, const float *X_dev, int64_t n,
-                          float *Y_dev, float *J_dev);
-/* One Adam step on a device mini-batch (X n x 10, Y n x 6); returns loss. */
+int mf_mlp_get_normalization(const mf_mlp *m, float *in_mean, float *in_std,
+                             float *out_mean, float *out_std);
+/* FP32 master weights of one member (mf_mlp_num_params floats).  Setting
+ * them resets that member's Adam moments. */
+int mf_mlp_get_weights(mf_mlp *m, int32_t member, float *w);
+int mf_mlp_set_weights(mf_mlp *m, int32_t member, const float *w);
+
+/* Training data and mini-batches live in HBM: X (n_data x 10), Y
+ * (n_data x 6), FP32.  idx_dev holds n_members x batch row indices (member k
+ * trains on rows idx[k*batch ...]); NULL means rows 0..batch-1 for every
+ * member.  batch is a positive multiple of MF_MLP_BATCH_ALIGN.  The loss is
+ * the mean squared error of the normalised outputs, averaged over members;
+ * passing loss != NULL waits for the step. */
 int mf_mlp_train_step_device(mf_mlp *m, const float *X_dev, const float *Y_dev,
-                             int64_t n, float lr, float *loss);
-/* Gradient of the mean-squared loss on a device mini-batch (no update);
- * grad has mf_mlp_num_params floats. */
-int mf_mlp_gradient_device(mf_mlp *m, const float *X_dev, const float *Y_dev,
-                           int64_t n, float *grad_host, float *loss);
+                             int64_t n_data, const int32_t *idx_dev,
+                             int32_t batch, float lr, float *loss);
+
+typedef struct mf_train_params {
+  int32_t epochs;
+  int32_t batch;         /* multiple of MF_MLP_BATCH_ALIGN                */
+  uint64_t seed;         /* shuffling: stream (seed, epoch*members+k)     */
+  float lr, lr_final;    /* cosine decay from lr to lr_final              */
+  float beta1, beta2, eps;
+} mf_train_params;
+void mf_default_train_params(mf_train_params *tp);
+/* Adam epochs over (X, Y) with a fresh permutation per member and epoch;
+ * epoch_loss (epochs x n_members, may be NULL) gets each epoch's mean loss. */
+int mf_mlp_fit_device(mf_mlp *m, const float *X_dev, const float *Y_dev,
+                      int64_t n, const mf_train_params *tp, float *epoch_loss);
+/* Gradient of the loss w.r.t. one member's packed parameters on rows
+ * 0..n-1 (n a multiple of MF_MLP_BATCH_ALIGN), no update. */
+int mf_mlp_gradient_device(mf_mlp *m, int32_t member, const float *X_dev,
+                           const float *Y_dev, int32_t n, float *grad_host,
+                           float *loss);
+
+/* Inference on device buffers.  member = -1 averages the ensemble, k >= 0
+ * runs member k alone.  Y_dev: n x 6.  J_dev (may be NULL): n x 36,
+ * J[s*36 + 6*i + j] = d y_i / d e_j, the input Jacobian w.r.t. the six
+ * strain inputs from the tangent-linear pass fused into the same kernel.
+ * spread_dev (may be NULL): n x 6 standard deviation of y over members. */
+int mf_mlp_predict_device(mf_mlp *m, int32_t member, const float *X_dev,
+                          int64_t n, float *Y_dev, float *J_dev,
+                          float *spread_dev);
+/* Kernel time (ms, HIP events) of the last predict. */
+int mf_mlp_last_predict_ms(const mf_mlp *m, double *ms);
 void *mf_mlp_stream(mf_mlp *m);
+
+/* ---- surrogate path with explicit fallback (no reference counterpart) ----
+ * Error estimator: a sample falls back to the explicit solver when any input
+ * lies outside [in_lo - margin*(in_hi-in_lo), in_hi + margin*(in_hi-in_lo)],
+ * or when kappa * max_i spread_i / max(max_i |y_i|, sigma_floor) > tol. */
+typedef struct mf_estimator {
+  float in_lo[MF_MLP_IN], in_hi[MF_MLP_IN];
+  float margin, kappa, sigma_floor, tol;
+} mf_estimator;
+int mf_mlp_set_estimator(mf_mlp *m, const mf_estimator *est);
+int mf_mlp_get_estimator(const mf_mlp *m, mf_estimator *est);
+
+/* The 10 surrogate inputs of every RVE of a batch for deformation F
+ * (n x 9, total deformation from the reference configuration): X n x 10. */
+int mf_surrogate_inputs(mf_batch *b, const double *F, int32_t nF, float *X);
+
+typedef struct mf_surrogate_stats {
+  int64_t n, n_fallback, n_out_of_range;
+  double mlp_ms;       /* input kernel + ensemble inference + estimator     */
+  double fallback_ms;  /* explicit solves of the rejected RVEs              */
+  double total_ms;     /* wall time of the call                             */
+} mf_surrogate_stats;
+/* Stress (and, when C != NULL, tangent) of every RVE of a ramp-mode batch
+ * whose state is the reference configuration (no update run): the ensemble
+ * predicts, the estimator rejects, rejected RVEs are solved by the explicit
+ * solver (trial runs; tangent by the central scheme with fd_step), which is
+ * also what the surrogate's outputs are compared against.  sigma n x 6,
+ * C n x 36 in the layout of mf_batch_tangent (the surrogate's Jacobian
+ * mapped to the perturbations F_i(h) F), status n (MF_STATUS_OK for accepted
+ * predictions, MF_STATUS_FALLBACK or the solver's status otherwise). */
+int mf_surrogate_run(mf_batch *b, mf_mlp *m, const double *F, int32_t nF,
+                     double *sigma, double *C, int32_t *status,
+                     mf_surrogate_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,26 @@ class mf_gen_params(C.Structure):
                 ("length_ratio_trns", C.c_double)]
 
 
+MF_MLP_IN, MF_MLP_HIDDEN, MF_MLP_OUT, MF_MLP_MAX_MEMBERS, MF_MLP_BATCH_ALIGN = 10, 256, 6, 8, 256
+
+
+class mf_train_params(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("seed", C.c_uint64),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class mf_estimator(C.Structure):
+    _fields_ = [("in_lo", C.c_float * MF_MLP_IN), ("in_hi", C.c_float * MF_MLP_IN),
+                ("margin", C.c_float), ("kappa", C.c_float), ("sigma_floor", C.c_float),
+                ("tol", C.c_float)]
+
+
+class mf_surrogate_stats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("n_fallback", C.c_int64), ("n_out_of_range", C.c_int64),
+                ("mlp_ms", C.c_double), ("fallback_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 # Every exported symbol of include/mumfim_amd.h with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int32
@@ -101,16 +121,25 @@ SIGNATURES = {
     "mf_measure_lds_bandwidth": (C.c_int, [_D]),
     "mf_measure_hbm_bandwidth": (C.c_int, [C.c_int64, _D]),
     "mf_load_network": (C.c_int, [C.c_char_p, _IP, _IP, _IP, _IP, _I, _D, _IP, _IP, _IP, C.POINTER(mf_reaction)]),
-    "mf_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    "mf_mlp_create": (C.c_int, [_I, C.c_uint64, C.POINTER(_P)]),
     "mf_mlp_destroy": (None, [_P]),
-    "mf_mlp_num_params": (C.c_int, [_P]),
+    "mf_mlp_num_params": (C.c_int32, []),
+    "mf_mlp_num_members": (C.c_int32, [_P]),
     "mf_mlp_set_normalization": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
-    "mf_mlp_get_weights": (C.c_int, [_P, _FP]),
-    "mf_mlp_set_weights": (C.c_int, [_P, _FP]),
-    "mf_mlp_forward_device": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
-    "mf_mlp_train_step_device": (C.c_int, [_P, _P, _P, C.c_int64, C.c_float, _FP]),
-    "mf_mlp_gradient_device": (C.c_int, [_P, _P, _P, C.c_int64, _FP, _FP]),
+    "mf_mlp_get_normalization": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
+    "mf_mlp_get_weights": (C.c_int, [_P, _I, _FP]),
+    "mf_mlp_set_weights": (C.c_int, [_P, _I, _FP]),
+    "mf_mlp_train_step_device": (C.c_int, [_P, _P, _P, C.c_int64, _P, _I, C.c_float, _FP]),
+    "mf_default_train_params": (None, [C.POINTER(mf_train_params)]),
+    "mf_mlp_fit_device": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mf_train_params), _FP]),
+    "mf_mlp_gradient_device": (C.c_int, [_P, _I, _P, _P, _I, _FP, _FP]),
+    "mf_mlp_predict_device": (C.c_int, [_P, _I, _P, C.c_int64, _P, _P, _P]),
+    "mf_mlp_last_predict_ms": (C.c_int, [_P, _D]),
     "mf_mlp_stream": (C.c_void_p, [_P]),
+    "mf_mlp_set_estimator": (C.c_int, [_P, C.POINTER(mf_estimator)]),
+    "mf_mlp_get_estimator": (C.c_int, [_P, C.POINTER(mf_estimator)]),
+    "mf_surrogate_inputs": (C.c_int, [_P, _D, _I, _FP]),
+    "mf_surrogate_run": (C.c_int, [_P, _P, _D, _I, _D, _D, _IP, C.POINTER(mf_surrogate_stats)]),
 }
 
 _lib = None
@@ -433,3 +462,142 @@ class BatchedFiberRVEAnalysisExplicit:
         p = np.zeros((self.n, 4), np.int32)
         ng = lib().mf_batch_plan(self.h, _iptr(p))
         return ng, p
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ptr(x):
+    return None if x is None else (x.data_ptr() if hasattr(x, "data_ptr") else int(x))
+
+
+def default_train_params(**kw):
+    tp = mf_train_params()
+    lib().mf_default_train_params(C.byref(tp))
+    for k, v in kw.items():
+        setattr(tp, k, v)
+    return tp
+
+
+class SurrogateMLP:
+    """Ensemble of BF16-MFMA MLPs 10 -> 256 -> 256 -> 256 -> 6 (SiLU) over the
+    C ABI (mf_mlp_*).  Inputs: strain [e_xx, e_yy, e_zz, g_yz, g_xz, g_xy]
+    of sym(F) - I, node count, fibers per volume, mean fiber length,
+    nonlinear fraction; outputs: Cauchy stress (Voigt)."""
+
+    NUM_PARAMS = 135942
+
+    def __init__(self, n_members=4, seed=1):
+        h = C.c_void_p()
+        check(lib().mf_mlp_create(int(n_members), int(seed), C.byref(h)))
+        self.h = h
+        self.n_members = int(n_members)
+
+    def close(self):
+        if self.h:
+            lib().mf_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_normalization(self, in_mean, in_std, out_mean, out_std):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (in_mean, in_std, out_mean, out_std)]
+        check(lib().mf_mlp_set_normalization(self.h, *[_fptr(v) for v in a]))
+
+    def normalization(self):
+        a = [np.zeros(MF_MLP_IN, np.float32), np.zeros(MF_MLP_IN, np.float32),
+             np.zeros(MF_MLP_OUT, np.float32), np.zeros(MF_MLP_OUT, np.float32)]
+        check(lib().mf_mlp_get_normalization(self.h, *[_fptr(v) for v in a]))
+        return a
+
+    def get_weights(self, member=0):
+        w = np.zeros(self.NUM_PARAMS, np.float32)
+        check(lib().mf_mlp_get_weights(self.h, int(member), _fptr(w)))
+        return w
+
+    def set_weights(self, member, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.NUM_PARAMS
+        check(lib().mf_mlp_set_weights(self.h, int(member), _fptr(w)))
+
+    def train_step(self, X_dev, Y_dev, n_data, batch, lr, idx_dev=None, return_loss=True):
+        loss = C.c_float()
+        check(lib().mf_mlp_train_step_device(self.h, _ptr(X_dev), _ptr(Y_dev), int(n_data),
+                                             _ptr(idx_dev), int(batch), float(lr),
+                                             C.byref(loss) if return_loss else None))
+        return loss.value if return_loss else None
+
+    def fit(self, X_dev, Y_dev, n, **kw):
+        """Adam epochs on device data; returns the epoch losses (epochs x members)."""
+        tp = default_train_params(**kw)
+        hist = np.zeros((tp.epochs, self.n_members), np.float32)
+        check(lib().mf_mlp_fit_device(self.h, _ptr(X_dev), _ptr(Y_dev), int(n), C.byref(tp),
+                                      _fptr(hist)))
+        return hist
+
+    def gradient(self, member, X_dev, Y_dev, n):
+        g = np.zeros(self.NUM_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().mf_mlp_gradient_device(self.h, int(member), _ptr(X_dev), _ptr(Y_dev),
+                                           int(n), _fptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def predict_device(self, X_dev, n, Y_dev, J_dev=None, spread_dev=None, member=-1):
+        check(lib().mf_mlp_predict_device(self.h, int(member), _ptr(X_dev), int(n),
+                                          _ptr(Y_dev), _ptr(J_dev), _ptr(spread_dev)))
+
+    def predict(self, X, jacobian=False, member=-1):
+        """Host convenience: returns (Y, spread, J or None)."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MF_MLP_IN)
+        n = len(X)
+        Xd = DeviceArray.from_host(X)
+        Yd = DeviceArray((n, MF_MLP_OUT), np.float32)
+        Sd = DeviceArray((n, MF_MLP_OUT), np.float32)
+        Jd = DeviceArray((n, 6, 6), np.float32) if jacobian else None
+        self.predict_device(Xd, n, Yd, Jd, Sd, member)
+        return Yd.numpy(), Sd.numpy(), (Jd.numpy() if jacobian else None)
+
+    def last_predict_ms(self):
+        v = C.c_double()
+        check(lib().mf_mlp_last_predict_ms(self.h, C.byref(v)))
+        return v.value
+
+    def set_estimator(self, in_lo, in_hi, margin=0.0, kappa=1.0, sigma_floor=0.0, tol=1e30):
+        e = mf_estimator()
+        for i in range(MF_MLP_IN):
+            e.in_lo[i] = float(in_lo[i])
+            e.in_hi[i] = float(in_hi[i])
+        e.margin, e.kappa, e.sigma_floor, e.tol = margin, kappa, sigma_floor, tol
+        check(lib().mf_mlp_set_estimator(self.h, C.byref(e)))
+
+    def estimator(self):
+        e = mf_estimator()
+        check(lib().mf_mlp_get_estimator(self.h, C.byref(e)))
+        return e
+
+
+def surrogate_inputs(analysis, F):
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+    X = np.zeros((analysis.n, MF_MLP_IN), np.float32)
+    check(lib().mf_surrogate_inputs(analysis.h, _dptr(F), F.shape[0], _fptr(X)))
+    return X
+
+
+def surrogate_run(analysis, mlp, F, tangent=False):
+    """Surrogate stress (and tangent) with explicit fallback for every RVE of
+    a ramp-mode batch at its reference state.  Returns sigma, C (or None),
+    status, stats dict."""
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+    sigma = np.zeros((analysis.n, 6))
+    Cm = np.zeros((analysis.n, 6, 6)) if tangent else None
+    status = np.zeros(analysis.n, np.int32)
+    st = mf_surrogate_stats()
+    check(lib().mf_surrogate_run(analysis.h, mlp.h, _dptr(F), F.shape[0], _dptr(sigma),
+                                 _dptr(Cm) if tangent else None, _iptr(status), C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in mf_surrogate_stats._fields_}
+    return sigma, Cm, status, stats

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -15,6 +16,8 @@
 
 #include "mf_internal.hpp"
 #include "mf_kernels.hpp"
+#include "dev_util.hpp"
+#include "mlp_internal.hpp"
 
 namespace mf {
 static thread_local std::string g_last_error;
@@ -23,44 +26,7 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 using namespace mf;
 
-#define MF_HIP(call)                                                   \
-  do {                                                                 \
-    hipError_t e_ = (call);                                            \
-    if (e_ != hipSuccess) {                                            \
-      set_error(std::string(#call) + ": " + hipGetErrorString(e_));    \
-      return MF_ERR_HIP;                                               \
-    }                                                                  \
-  } while (0)
-
 namespace {
-
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * count);
-  }
-  hipError_t upload(const std::vector<T> &v, hipStream_t s) {
-    hipError_t e = alloc(v.size());
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(),
-                          hipMemcpyHostToDevice, s);
-  }
-  hipError_t ensure(size_t count) {
-    if (count <= n && p) return hipSuccess;
-    return alloc(count);
-  }
-};
 
 void make_react_dev(const Reaction &q, ReactDev &d) {
   std::memset(&d, 0, sizeof(d));
@@ -199,6 +165,12 @@ struct mf_batch {
   std::vector<size_t> id_group_lds[2];              // LDS bytes per group
   DevBuf<double> lastF_dev;
   bool lastF_on_device = false;
+  // state is the reference configuration (no update run, no set_state)
+  bool at_reference = true;
+  // surrogate path buffers
+  DevBuf<float> sur_desc, sur_X, sur_Y, sur_J, sur_spread;
+  DevBuf<double> sur_sigma, sur_C;
+  DevBuf<int32_t> sur_flag;
   ~mf_batch() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -654,10 +626,12 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
                           b->id_group_lds[prec]);
     if (e != MF_OK) return e;
   }
+  if (!ramp) b->at_reference = false;  // relax runs keep the trial x (Q5)
   if (update) {
     MF_HIP(hipMemcpyAsync(b->cur_stress.p, a.sigma, sizeof(double) * 6 * R,
                           hipMemcpyDeviceToDevice, s));
     b->have_run = true;
+    b->at_reference = false;
     if (ramp) {
       b->cur ^= 1;
       b->have_prev = true;
@@ -1047,6 +1021,7 @@ int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
   MF_HIP(hipStreamSynchronize(s));
   b->have_run = true;
   b->have_prev = true;
+  b->at_reference = false;
   b->cur ^= 1;
   b->lastF_on_device = false;
   b->lastF.assign(9 * (size_t)R, 0.0);
@@ -1150,6 +1125,7 @@ int mf_batch_set_state(mf_batch *b, const double *x, const double *u,
                           hipMemcpyHostToDevice, s));
   MF_HIP(hipStreamSynchronize(s));
   b->have_prev = false;
+  b->at_reference = false;
   return MF_OK;
 }
 
@@ -1305,6 +1281,179 @@ int mf_load_network(const char *path, int32_t *n_nodes, int32_t *n_fibers,
       r.length_ratio_trns = q.lt;
       r.cmp_ratio_trns = q.lc;
     }
+  }
+  return MF_OK;
+}
+
+
+// ---- surrogate path --------------------------------------------------------
+
+static int surrogate_prepare(mf_batch *b, const double *F, int32_t nF) {
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  if (!b->sur_desc.p) {
+    // node count, fibers per volume (V = 1 at the reference state), mean
+    // fiber length, nonlinear-fiber fraction
+    std::vector<float> d(4 * (size_t)R);
+    for (int r = 0; r < R; ++r) {
+      const PackedRVE &p = b->host[r];
+      d[4 * r + 0] = (float)p.N;
+      d[4 * r + 1] = (float)p.M;
+      d[4 * r + 2] = (float)p.mean_l0;
+      d[4 * r + 3] = (float)p.nonlinear_frac;
+    }
+    MF_HIP(b->sur_desc.upload(d, s));
+  }
+  MF_HIP(b->F_in.ensure(9 * (size_t)nF));
+  MF_HIP(hipMemcpyAsync(b->F_in.p, F, sizeof(double) * 9 * nF, hipMemcpyHostToDevice, s));
+  MF_HIP(b->sur_X.ensure((size_t)R * MF_MLP_IN));
+  MF_HIP(launch_surrogate_inputs(b->F_in.p, nF, b->sur_desc.p, R, b->sur_X.p, s));
+  return MF_OK;
+}
+
+int mf_surrogate_inputs(mf_batch *b, const double *F, int32_t nF, float *X) {
+  if (!b || !F || !X) {
+    set_error("mf_surrogate_inputs: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  int e = check_F(b, nF, 0);
+  if (e != MF_OK) return e;
+  if ((e = surrogate_prepare(b, F, nF)) != MF_OK) return e;
+  MF_HIP(hipMemcpyAsync(X, b->sur_X.p, sizeof(float) * MF_MLP_IN * b->R,
+                        hipMemcpyDeviceToHost, b->stream));
+  MF_HIP(hipStreamSynchronize(b->stream));
+  return MF_OK;
+}
+
+int mf_surrogate_run(mf_batch *b, mf_mlp *m, const double *F, int32_t nF, double *sigma,
+                     double *C, int32_t *status, mf_surrogate_stats *stats) {
+  const auto t_start = std::chrono::steady_clock::now();
+  if (!b || !m || !F || !sigma) {
+    set_error("mf_surrogate_run: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  if (b->p.mode != MF_MODE_RAMP || !b->at_reference) {
+    set_error("mf_surrogate_run: needs a ramp-mode batch in its reference configuration "
+              "(the surrogate maps total strain from the reference state)");
+    return MF_ERR_STATE;
+  }
+  int e = check_F(b, nF, 0);
+  if (e != MF_OK) return e;
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  const int K = mlp_members(m);
+  MF_HIP(hipEventRecord(b->ev0, s));
+  if ((e = surrogate_prepare(b, F, nF)) != MF_OK) return e;
+  MF_HIP(b->sur_Y.ensure((size_t)R * MF_MLP_OUT));
+  MF_HIP(b->sur_spread.ensure((size_t)R * MF_MLP_OUT));
+  MF_HIP(b->sur_flag.ensure(R));
+  MF_HIP(b->sur_sigma.ensure(6 * (size_t)R));
+  if (C) {
+    MF_HIP(b->sur_J.ensure(36 * (size_t)R));
+    MF_HIP(b->sur_C.ensure(36 * (size_t)R));
+  }
+  float *spread = K > 1 ? b->sur_spread.p : nullptr;
+  if ((e = mlp_predict(m, -1, b->sur_X.p, R, b->sur_Y.p, C ? b->sur_J.p : nullptr, spread, s)) !=
+      MF_OK)
+    return e;
+  MF_HIP(launch_surrogate_flags(b->sur_X.p, b->sur_Y.p, spread, R, mlp_estimator(m),
+                                b->sur_flag.p, s));
+  MF_HIP(launch_surrogate_outputs(b->F_in.p, nF, b->sur_Y.p, C ? b->sur_J.p : nullptr, R,
+                                  b->sur_sigma.p, C ? b->sur_C.p : nullptr, s));
+  MF_HIP(hipEventRecord(b->ev1, s));
+  std::vector<int32_t> flag(R);
+  MF_HIP(hipMemcpyAsync(flag.data(), b->sur_flag.p, sizeof(int32_t) * R, hipMemcpyDeviceToHost,
+                        s));
+  MF_HIP(hipMemcpyAsync(sigma, b->sur_sigma.p, sizeof(double) * 6 * R, hipMemcpyDeviceToHost,
+                        s));
+  if (C)
+    MF_HIP(hipMemcpyAsync(C, b->sur_C.p, sizeof(double) * 36 * R, hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  float mlp_ms = 0;
+  MF_HIP(hipEventElapsedTime(&mlp_ms, b->ev0, b->ev1));
+  std::vector<int32_t> rej;
+  int64_t n_oor = 0;
+  for (int r = 0; r < R; ++r) {
+    if (status) status[r] = MF_STATUS_OK;
+    if (flag[r]) rej.push_back(r);
+    if (flag[r] == 2) ++n_oor;
+  }
+  float fb_ms = 0;
+  if (!rej.empty()) {
+    // explicit trial solves of the rejected RVEs from the reference state;
+    // with C: base + (F_i(+h) F, F_i(-h) F) for i = 0..5, 13 per RVE
+    const int per = C ? 13 : 1;
+    const int nr = (int)rej.size();
+    SolveSet S;
+    S.prec = C ? b->p.tangent_precision : b->p.precision;
+    S.rve.resize((size_t)nr * per);
+    S.flags.assign((size_t)nr * per, 0);
+    S.F.resize(9 * (size_t)nr * per);
+    const double h = b->p.fd_step;
+    for (int k = 0; k < nr; ++k) {
+      const int r = rej[k];
+      const double *Fr = F + 9 * (nF == 1 ? 0 : r);
+      for (int q = 0; q < per; ++q) {
+        S.rve[(size_t)k * per + q] = r;
+        double *dst = S.F.data() + 9 * ((size_t)k * per + q);
+        if (q == 0) {
+          std::copy(Fr, Fr + 9, dst);
+        } else {
+          double Fp[9];
+          perturbation((q - 1) / 2, (q - 1) % 2 ? -h : h, Fp);
+          matmul3(Fp, Fr, dst);
+        }
+      }
+    }
+    DevBuf<double> sig;
+    DevBuf<int32_t> st;
+    MF_HIP(sig.alloc(6 * S.rve.size()));
+    MF_HIP(st.alloc(S.rve.size()));
+    MF_HIP(hipEventRecord(b->ev0, s));
+    const int c = b->cur;
+    if ((e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c].p, b->us[c].p, sig.p, st.p)) !=
+        MF_OK)
+      return e;
+    DevBuf<double> Cd;
+    if (C) {
+      MF_HIP(Cd.alloc(36 * (size_t)nr));
+      MF_HIP(launch_fd_all(sig.p, Cd.p, nr, MF_TANGENT_CENTRAL, h, s));
+    }
+    MF_HIP(hipEventRecord(b->ev1, s));
+    std::vector<double> sh(6 * S.rve.size()), ch(C ? 36 * (size_t)nr : 0);
+    std::vector<int32_t> sth(S.rve.size());
+    MF_HIP(hipMemcpyAsync(sh.data(), sig.p, sizeof(double) * sh.size(), hipMemcpyDeviceToHost,
+                          s));
+    MF_HIP(hipMemcpyAsync(sth.data(), st.p, sizeof(int32_t) * sth.size(),
+                          hipMemcpyDeviceToHost, s));
+    if (C)
+      MF_HIP(hipMemcpyAsync(ch.data(), Cd.p, sizeof(double) * ch.size(), hipMemcpyDeviceToHost,
+                            s));
+    MF_HIP(hipStreamSynchronize(s));
+    MF_HIP(hipEventElapsedTime(&fb_ms, b->ev0, b->ev1));
+    for (int k = 0; k < nr; ++k) {
+      const int r = rej[k];
+      std::copy(sh.begin() + 6 * (size_t)k * per, sh.begin() + 6 * (size_t)k * per + 6,
+                sigma + 6 * (size_t)r);
+      if (C) std::copy(ch.begin() + 36 * (size_t)k, ch.begin() + 36 * (size_t)k + 36,
+                       C + 36 * (size_t)r);
+      if (status) {
+        int sv = MF_STATUS_FALLBACK;
+        for (int q = 0; q < per; ++q)
+          if (sth[(size_t)k * per + q] != MF_STATUS_OK) sv = sth[(size_t)k * per + q];
+        status[r] = sv;
+      }
+    }
+  }
+  if (stats) {
+    stats->n = R;
+    stats->n_fallback = (int64_t)rej.size();
+    stats->n_out_of_range = n_oor;
+    stats->mlp_ms = mlp_ms;
+    stats->fallback_ms = fb_ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t_start)
+                          .count();
   }
   return MF_OK;
 }
