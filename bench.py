@@ -199,6 +199,153 @@ def bench_cfg4(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_cfg5(args):
+    """Learned surrogate.  Training data: explicit FP32c solves of config 2's
+    generator (500 nodes, k=4, linear fibers, 5000 ramp steps) with
+    F = I + sym(U), U ~ U(-0.1, 0.1): 65536 networks x 16 strains = 1,048,576
+    solves.  An ensemble of 4 MLPs is trained on the GPU (BF16 MFMA, Adam).
+    Inference is timed on 2^20 points (stress, and stress + tangent).  A
+    held-out batch of 16384 new networks calibrates the error estimator, a
+    second one measures the surrogate path (ensemble, estimator, explicit
+    fallback) against the FP64 solver."""
+    import mumfim_amd as mf
+    from oracle import mlp_oracle as mo
+    from oracle.oracle import Oracle
+    n_net, per, S = args.surrogate_networks, 16, 5000
+    n_data = n_net * per
+    base = dict(mode=mf.MF_MODE_RAMP, n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER)
+    p32 = mf.default_params(precision=mf.MF_PREC_FP32C, **base)
+    t = time.perf_counter()
+    g = mf.BatchedFiberRVEAnalysisExplicit(params=p32, generated=(mf.default_gen_params(seed=1),
+                                                                 n_net))
+    log(f"[cfg5] {n_net} networks built in {time.perf_counter() - t:.1f}s")
+    X = np.empty((n_data, mf.MF_MLP_IN), np.float32)
+    Y = np.empty((n_data, 6), np.float32)
+    F_dev = mf.DeviceArray((n_net, 9))
+    sig_dev = mf.DeviceArray((n_net, 6))
+    solve_s = 0.0
+    for k in range(per):
+        F = strain_F(1000 + k, n_net, 0.1)
+        F_dev.copy_from(F.reshape(n_net, 9))
+        t = time.perf_counter()
+        g.run_device(F_dev, sig_dev)
+        g.synchronize()
+        solve_s += time.perf_counter() - t
+        Y[k * n_net:(k + 1) * n_net] = sig_dev.numpy()
+        X[k * n_net:(k + 1) * n_net] = mf.surrogate_inputs(g, F)
+        log(f"[cfg5] data {k + 1}/{per}: {n_net * (k + 1)} solves, {solve_s:.1f}s solving")
+    explicit_rate = n_data / solve_s
+    g.close()
+    del F_dev, sig_dev
+
+    # training on the GPU
+    Xd, Yd = mf.DeviceArray.from_host(X), mf.DeviceArray.from_host(Y)
+    m = mf.SurrogateMLP(n_members=4, seed=7)
+    m.set_normalization(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
+    tp = dict(epochs=args.surrogate_epochs, batch=4096, lr=2e-3, lr_final=1e-5, seed=8)
+    mf.device_synchronize()
+    t = time.perf_counter()
+    hist = m.fit(Xd, Yd, n_data, **tp)
+    train_s = time.perf_counter() - t
+    log(f"[cfg5] trained {tp['epochs']} epochs in {train_s:.1f}s, loss "
+        f"{hist[0].mean():.4f} -> {hist[-1].mean():.4f}")
+    # gradient of one member vs the CPU oracle on 512 training rows
+    nm = m.normalization()
+    norm = mo.Norm(*nm)
+    _, gg = m.gradient(0, Xd, Yd, 512)
+    _, go = mo.loss_and_grad(m.get_weights(0), norm, X[:512], Y[:512], emulate=True)
+    grad_err = float(np.linalg.norm(gg - go) / np.linalg.norm(go))
+
+    # inference throughput on 2^20 points
+    npts = min(1 << 20, n_data)
+    Yp = mf.DeviceArray((npts, 6), np.float32)
+    Sp = mf.DeviceArray((npts, 6), np.float32)
+    Jp = mf.DeviceArray((npts, 6, 6), np.float32)
+    t_sig, t_jac = [], []
+    for rep in range(1 + args.steps):
+        m.predict_device(Xd, npts, Yp, None, Sp)
+        if rep:
+            t_sig.append(m.last_predict_ms())
+        m.predict_device(Xd, npts, Yp, Jp, Sp)
+        if rep:
+            t_jac.append(m.last_predict_ms())
+    ms_sig, ms_jac = float(np.mean(t_sig)), float(np.mean(t_jac))
+    flop_fwd = 2 * (mf.MF_MLP_IN * 256 + 2 * 256 * 256 + 256 * 6) * 4  # per point, 4 members
+    del Xd, Yd, Yp, Sp, Jp
+
+    # estimator calibration on held-out networks (FP32c explicit reference)
+    nv = 16384
+    gv = mf.BatchedFiberRVEAnalysisExplicit(
+        params=p32, generated=(mf.default_gen_params(seed=10_000_001), nv))
+    Fv = strain_F(2001, nv, 0.1)
+    sv = gv.run(Fv, update_coords=False)
+    Xv = mf.surrogate_inputs(gv, Fv)
+    yv, spv, _ = m.predict(Xv)
+    ev = rel_err(yv.astype(np.float64), sv)
+    qv = spv.max(1) / np.abs(yv).max(1)
+    kappa = float(np.median(ev / qv))
+    gv.close()
+
+    # test: held-out networks, surrogate path vs the FP64 solver
+    nt = 16384
+    gen_t = mf.default_gen_params(seed=20_000_001)
+    gt = mf.BatchedFiberRVEAnalysisExplicit(params=p32, generated=(gen_t, nt))
+    Ft = strain_F(3001, nt, 0.1)
+    gt.set_params(precision=mf.MF_PREC_FP64)
+    s64 = gt.run(Ft, update_coords=False)
+    t64_ms = gt.last_solve_ms()[0]
+    gt.set_params(precision=mf.MF_PREC_FP32C)
+    lo, hi = X.min(0), X.max(0)
+    rows = []
+    for tau in (0.1, 0.2, 0.3, 0.5, 1e30):
+        m.set_estimator(lo, hi, margin=0.02, kappa=kappa, sigma_floor=0.0, tol=tau)
+        sig, _, st, stt = mf.surrogate_run(gt, m, Ft)
+        e = rel_err(sig, s64)
+        acc = st == mf.MF_STATUS_OK
+        rate = stt["n_fallback"] / nt
+        # effective rate on 2^20 points: ensemble time + explicit solves of the
+        # rejected share at the explicit rate measured above
+        t_1m = ms_sig / 1e3 + rate * npts / explicit_rate
+        rows.append({"tol": None if tau > 1e29 else tau, "fallback_rate": round(rate, 4),
+                     "out_of_range": int(stt["n_out_of_range"]),
+                     "err_accepted_median": float(np.median(e[acc])) if acc.any() else None,
+                     "err_accepted_p90": float(np.percentile(e[acc], 90)) if acc.any() else None,
+                     "err_all_median": float(np.median(e)),
+                     "err_all_p90": float(np.percentile(e, 90)),
+                     "path_ms_16384": round(stt["total_ms"], 3),
+                     "points_per_s_1M": round(npts / t_1m, 1),
+                     "speedup_vs_explicit": round(npts / t_1m / explicit_rate, 2)})
+        log(f"[cfg5] tol {tau:g}: {rows[-1]}")
+    res = {"config": "cfg5",
+           "workload": f"ensemble of 4 MLPs 10-256-256-256-6 (SiLU, BF16 MFMA) trained on "
+                       f"{n_data} FP32c solves ({n_net} networks of 500 nodes x {per} strains, "
+                       f"U~U(-0.1,0.1), {S} ramp steps); inference on {npts} points; "
+                       f"surrogate path on {nt} held-out networks vs the FP64 solver",
+           "metric": "surrogate points/s (stress) on 2^20 points incl. fallback",
+           "value": rows[2]["points_per_s_1M"], "unit": "points/s",
+           "dtype": "bf16 (MFMA, FP32 accumulate)",
+           "explicit_solves_per_s": round(explicit_rate, 1),
+           "fp64_solves_per_s_test": round(nt / (t64_ms / 1e3), 1),
+           "train_s": round(train_s, 2), "epochs": tp["epochs"],
+           "loss_first_last": [float(hist[0].mean()), float(hist[-1].mean())],
+           "grad_rel_err_vs_oracle": grad_err,
+           "infer_ms_1M": round(ms_sig, 3), "infer_ms_1M_with_tangent": round(ms_jac, 3),
+           "infer_points_per_s": round(npts / ms_sig * 1e3, 1),
+           "infer_points_per_s_with_tangent": round(npts / ms_jac * 1e3, 1),
+           "infer_tflops": round(flop_fwd * npts / (ms_sig / 1e3) / 1e12, 2),
+           "infer_tflops_with_tangent": round(7 * flop_fwd * npts / (ms_jac / 1e3) / 1e12, 2),
+           "estimator_kappa": kappa,
+           "validation_err_median": float(np.median(ev)),
+           "by_tolerance": rows}
+    if not args.no_cpu:
+        n = 16
+        nets = [mf.generate_network(seed=20_000_001 + r, n_nodes=500) for r in range(n)]
+        o = Oracle(nets, mf.default_params(precision=mf.MF_PREC_FP64, **base))
+        so = o.run(Ft[:n], update_coords=False)
+        res["fp64_gpu_vs_oracle_max"] = float(rel_err(s64[:n], so).max())
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,14 +358,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
-    ap.add_argument("--config", choices=["cfg2", "cfg3", "cfg4"], default="cfg2",
+    ap.add_argument("--config", choices=["cfg2", "cfg3", "cfg4", "cfg5"], default="cfg2",
                     help="cfg2 (default): 4096 x 500 nodes; cfg3: 16384 RVEs of 200-4000 "
                          "nodes, mixed reactions, 10000 steps; cfg4: 4096 RVEs of "
-                         "1000-2000 nodes, base + 12 perturbed solves per RVE")
+                         "1000-2000 nodes, base + 12 perturbed solves per RVE; cfg5: learned "
+                         "surrogate trained on 1M solves, inference on 1M points")
+    ap.add_argument("--surrogate-networks", type=int, default=65536,
+                    help="cfg5: networks in the training data (16 strains each)")
+    ap.add_argument("--surrogate-epochs", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (RVEs shard by RVE across GPUs, see DESIGN.md)", file=sys.stderr)
+              "supported (a multi-GPU run would shard the batch by RVE)", file=sys.stderr)
         sys.exit(2)
 
     import mumfim_amd as mf
@@ -227,6 +378,8 @@ def main():
         return bench_cfg3(args)
     if args.config == "cfg4":
         return bench_cfg4(args)
+    if args.config == "cfg5":
+        return bench_cfg5(args)
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])
