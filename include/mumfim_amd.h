@@ -141,6 +141,11 @@ void mf_batch_destroy(mf_batch *b);
 int mf_batch_num_rves(const mf_batch *b);
 int mf_batch_set_params(mf_batch *b, const mf_solver_params *params);
 int mf_batch_get_params(const mf_batch *b, mf_solver_params *params);
+/* Per-RVE viscous damping coefficients (n_rve doubles), as the reference
+ * keeps one per RVE from its solution strategy
+ * (BatchedFiberRVEAnalysisExplicit.h:213-218); NULL restores
+ * params.visc_damp_coeff for every RVE. */
+int mf_batch_set_damping(mf_batch *b, const double *eta);
 
 /* run(F, sigma, update_coords): host buffers. */
 int mf_batch_run(mf_batch *b, const double *F, int32_t nF, double *sigma,
@@ -237,6 +242,59 @@ int mf_load_network(const char *path, int32_t *n_nodes, int32_t *n_fibers,
                     int32_t max_reactions, double *coords, int32_t *fibers,
                     int32_t *fiber_reaction, int32_t *boundary,
                     mf_reaction *reactions);
+
+/* ---- FE^2 message adapter (MultiscaleRVEAnalysis.cc:197-305) -------------
+ * The micro side of the multiscale loop.  Per-RVE records as the macro scale
+ * sends them (MultiscaleMicroFOParams.h:19-120), row-major, one per RVE:
+ *   hdrs      7 int32  micro_fo_header: RVE_TYPE, RVE_DIR_TYPE, FIELD_ORDER,
+ *                      ELEMENT_TYPE, GAUSS_ID, COMPUTE_ORIENTATION_3D/2D
+ *   prms      8 double micro_fo_params: FIBER_RADIUS, VOLUME_FRACTION,
+ *                      YOUNGS_MODULUS, NONLINEAR_PARAM, LINEAR_TRANSITION,
+ *                      ORIENTATION_AXIS_X/Y/Z
+ *   slvr      8 double micro_fo_solver: ..., VISCOUS_DAMPING_FACTOR (5),
+ *                      CRITICAL_TIME_SCALE_FACTOR (6), ...
+ *   slvr_int 10 int32  micro_fo_int_solver: MICRO_SOLVER_TYPE (0), ...
+ *   data      9 double micro_fo_data: deformation gradient, row-major
+ *   results  45 double micro_fo_result: sigma (6), 0 (3), C (36)
+ *   step     18 double micro_fo_step_result: Omega_3D (9), Omega_2D (9)
+ * The networks themselves are the caller's (the reference picks a file per
+ * RVE from its network library); each gets scale factor
+ * mf_fo_scale_conversion(net, FIBER_RADIUS, VOLUME_FRACTION) and damping
+ * VISCOUS_DAMPING_FACTOR.  CRITICAL_TIME_SCALE_FACTOR is not used: the
+ * reference's batched loop fixes it at 0.8
+ * (BatchedFiberRVEAnalysisExplicitTeamOuterLoop.h:312). */
+#define MF_FO_HEADER_FIELDS 7
+#define MF_FO_PARAM_FIELDS 8
+#define MF_FO_SOLVER_FIELDS 8
+#define MF_FO_SOLVER_INT_FIELDS 10
+#define MF_FO_DATA 9
+#define MF_FO_RESULT 45
+#define MF_FO_STEP_RESULT 18
+#define MF_FO_FIBER_ONLY 1          /* MicroscaleType::FIBER_ONLY          */
+#define MF_FO_ISOTROPIC_NEOHOOKEAN 3 /* MicroscaleType::ISOTROPIC_NEOHOOKEAN */
+#define MF_FO_SOLVER_EXPLICIT 1     /* SolverType::Explicit                */
+
+/* calcScaleConversion (MultiscaleCoupling.cc:7-22): 1 / rve_dim^2 with
+ * rve_dim = sqrt(total fiber length * pi r^2 / volume fraction). */
+int mf_fo_scale_conversion(const mf_network *net, double fiber_radius,
+                           double volume_fraction, double *scale);
+
+typedef struct mf_fo mf_fo;
+/* updateCoupling (MultiscaleRVEAnalysis.cc:102-196) for n RVEs; params may
+ * be NULL (reference defaults). */
+int mf_fo_create(const mf_network *nets, int32_t n, const int32_t *hdrs,
+                 const double *prms, const double *slvr,
+                 const int32_t *slvr_int, const mf_solver_params *params,
+                 mf_fo **out);
+void mf_fo_destroy(mf_fo *f);
+/* The batch behind the adapter (owned by it). */
+mf_batch *mf_fo_batch(mf_fo *f);
+/* One macro iteration: run(F) with update, then the tangent
+ * (MultiscaleRVEAnalysis.cc:233-288).  data n x 9, results n x 45. */
+int mf_fo_run(mf_fo *f, const double *data, double *results, int32_t *status);
+/* End of a macro step: orientation tensors with the normals of
+ * ORIENTATION_AXIS_X/Y/Z (MultiscaleCoupling.h:32-52).  step n x 18. */
+int mf_fo_step_results(mf_fo *f, double *step_results);
 
 /* ---- learned surrogate (no reference counterpart) -------------------------
  * An ensemble of MLPs 10 -> 256 -> 256 -> 256 -> 6 with SiLU activations.

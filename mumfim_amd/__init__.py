@@ -139,6 +139,14 @@ SIGNATURES = {
     "mf_mlp_set_estimator": (C.c_int, [_P, C.POINTER(mf_estimator)]),
     "mf_mlp_get_estimator": (C.c_int, [_P, C.POINTER(mf_estimator)]),
     "mf_surrogate_inputs": (C.c_int, [_P, _D, _I, _FP]),
+    "mf_batch_set_damping": (C.c_int, [_P, _D]),
+    "mf_fo_scale_conversion": (C.c_int, [C.POINTER(mf_network), C.c_double, C.c_double, _D]),
+    "mf_fo_create": (C.c_int, [C.POINTER(mf_network), _I, _IP, _D, _D, _IP,
+                               C.POINTER(mf_solver_params), C.POINTER(_P)]),
+    "mf_fo_destroy": (None, [_P]),
+    "mf_fo_batch": (C.c_void_p, [_P]),
+    "mf_fo_run": (C.c_int, [_P, _D, _D, _IP]),
+    "mf_fo_step_results": (C.c_int, [_P, _D]),
     "mf_surrogate_run": (C.c_int, [_P, _P, _D, _I, _D, _D, _IP, C.POINTER(mf_surrogate_stats)]),
 }
 
@@ -384,6 +392,14 @@ class BatchedFiberRVEAnalysisExplicit:
         except Exception:
             pass
 
+    def set_damping(self, eta=None):
+        """Per-RVE viscous damping (None: params.visc_damp_coeff for all)."""
+        if eta is None:
+            check(lib().mf_batch_set_damping(self.h, None))
+        else:
+            e = np.ascontiguousarray(eta, dtype=np.float64).reshape(self.n)
+            check(lib().mf_batch_set_damping(self.h, _dptr(e)))
+
     def set_params(self, **kw):
         for k, v in kw.items():
             setattr(self.params, k, v)
@@ -601,3 +617,59 @@ def surrogate_run(analysis, mlp, F, tangent=False):
                                  _dptr(Cm) if tangent else None, _iptr(status), C.byref(st)))
     stats = {k: getattr(st, k) for k, _ in mf_surrogate_stats._fields_}
     return sigma, Cm, status, stats
+
+
+MF_FO_HEADER_FIELDS, MF_FO_PARAM_FIELDS, MF_FO_SOLVER_FIELDS, MF_FO_SOLVER_INT_FIELDS = 7, 8, 8, 10
+MF_FO_FIBER_ONLY, MF_FO_ISOTROPIC_NEOHOOKEAN, MF_FO_SOLVER_EXPLICIT = 1, 3, 1
+
+
+def fo_scale_conversion(network, fiber_radius, volume_fraction):
+    """calcScaleConversion (MultiscaleCoupling.cc:7-22) for one network."""
+    v = C.c_double()
+    d = network.to_c()
+    check(lib().mf_fo_scale_conversion(C.byref(d), float(fiber_radius), float(volume_fraction),
+                                       C.byref(v)))
+    return v.value
+
+
+class MultiscaleRVEAdapter:
+    """The micro side of MultiscaleRVEAnalysis::run over the batch: per-RVE
+    micro_fo_* records in, micro_fo_result (45 doubles) per macro iteration
+    and micro_fo_step_result (18 doubles) per macro step out."""
+
+    def __init__(self, networks, hdrs, prms, slvr, slvr_int, params=None):
+        n = len(networks)
+        self._nets = (mf_network * n)(*[x.to_c() for x in networks])
+        self._keep = networks
+        h = np.ascontiguousarray(hdrs, dtype=np.int32).reshape(n, MF_FO_HEADER_FIELDS)
+        p = np.ascontiguousarray(prms, dtype=np.float64).reshape(n, MF_FO_PARAM_FIELDS)
+        s = np.ascontiguousarray(slvr, dtype=np.float64).reshape(n, MF_FO_SOLVER_FIELDS)
+        si = np.ascontiguousarray(slvr_int, dtype=np.int32).reshape(n, MF_FO_SOLVER_INT_FIELDS)
+        hp = C.c_void_p()
+        check(lib().mf_fo_create(self._nets, n, _iptr(h), _dptr(p), _dptr(s), _iptr(si),
+                                 C.byref(params) if params is not None else None, C.byref(hp)))
+        self.h = hp
+        self.n = n
+
+    def close(self):
+        if self.h:
+            lib().mf_fo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, data, return_status=False):
+        d = np.ascontiguousarray(data, dtype=np.float64).reshape(self.n, 9)
+        res = np.zeros((self.n, 45))
+        st = np.zeros(self.n, np.int32)
+        check(lib().mf_fo_run(self.h, _dptr(d), _dptr(res), _iptr(st)))
+        return (res, st) if return_status else res
+
+    def step_results(self):
+        out = np.zeros((self.n, 18))
+        check(lib().mf_fo_step_results(self.h, _dptr(out)))
+        return out

@@ -23,6 +23,7 @@ DEVICE = [f"--offload-arch={ARCH}",
 SOURCES = {
     "network.cpp": [],
     "api.cpp": [],
+    "fe2.cpp": [],
     "aux_kernels.hip": ["-ffp-contract=off"],
     "solve64.hip": ["-ffp-contract=off"],
     "solve32.hip": [],

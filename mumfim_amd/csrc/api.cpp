@@ -167,6 +167,8 @@ struct mf_batch {
   bool lastF_on_device = false;
   // state is the reference configuration (no update run, no set_state)
   bool at_reference = true;
+  // per-RVE viscous damping (empty: visc_damp_coeff for all)
+  DevBuf<double> eta_rve;
   // surrogate path buffers
   DevBuf<float> sur_desc, sur_X, sur_Y, sur_J, sur_spread;
   DevBuf<double> sur_sigma, sur_C;
@@ -196,7 +198,16 @@ int make_plan(mf_batch *b, int prec) {
     const PackedRVE &p = b->host[r];
     int best = -1;
     long best_key = 0;
-    const bool lin = p.reactions.size() == 1 && p.reactions[0].type == MF_REACTION_LINEAR;
+    // linear-only kernels take EA from reaction 0: every fiber must use a
+    // linear reaction with that same E*A
+    bool lin = !p.reactions.empty() && p.reactions[0].type == MF_REACTION_LINEAR;
+    if (lin) {
+      const double ea0 = p.reactions[0].E * p.reactions[0].area();
+      for (int e = 0; e < p.M && lin; ++e) {
+        const Reaction &q = p.reactions[p.rid[e]];
+        lin = q.type == MF_REACTION_LINEAR && q.E * q.area() == ea0;
+      }
+    }
     for (int i = 0; i < nv; ++i) {
       const Variant &v = vs[i];
       if (v.prec != prec) continue;
@@ -438,6 +449,7 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.max_steps = b->p.max_steps;
   a.freq = b->p.dt_update_freq;
   a.eta = b->p.visc_damp_coeff;
+  a.eta_rve = b->eta_rve.p;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
   std::vector<std::pair<int, int>> groups;
@@ -477,6 +489,7 @@ void fill_args(mf_batch *b, SolveArgs &a) {
   a.max_steps = b->p.max_steps;
   a.freq = b->p.dt_update_freq;
   a.eta = b->p.visc_damp_coeff;
+  a.eta_rve = b->eta_rve.p;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
 }
@@ -867,6 +880,26 @@ int mf_batch_set_params(mf_batch *b, const mf_solver_params *params) {
       if (e != MF_OK) return e;
     }
   return build_identity_tables(b);
+}
+
+int mf_batch_set_damping(mf_batch *b, const double *eta) {
+  if (!b) {
+    set_error("mf_batch_set_damping: NULL batch");
+    return MF_ERR_INVALID;
+  }
+  if (!eta) {
+    b->eta_rve.release();
+    return MF_OK;
+  }
+  for (int r = 0; r < b->R; ++r)
+    if (!(eta[r] >= 0.0) || !std::isfinite(eta[r])) {
+      set_error("mf_batch_set_damping: damping must be finite and >= 0");
+      return MF_ERR_INVALID;
+    }
+  std::vector<double> v(eta, eta + b->R);
+  MF_HIP(b->eta_rve.upload(v, b->stream));
+  MF_HIP(hipStreamSynchronize(b->stream));
+  return MF_OK;
 }
 
 int mf_batch_get_params(const mf_batch *b, mf_solver_params *params) {

@@ -37,6 +37,9 @@ struct SolveArgs {
   int32_t mode;
   int32_t n_steps, max_steps, freq;
   double eta, crit, tol;
+  // per-RVE viscous damping (BatchedFiberRVEAnalysisExplicit.h:213-218);
+  // NULL: eta for every RVE
+  const double *eta_rve;
 };
 
 // A kernel variant: threads per solve, free-node slots per thread and the

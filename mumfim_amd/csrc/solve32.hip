@@ -108,7 +108,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       }
     }
   };
-  const float eta = (float)a.eta;
+  const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
 #pragma unroll
   for (int s = 0; s < NPT; ++s) {
     const int r = s * B + tid;
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
-  const float eta = (float)a.eta;
+  const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
   int cur = 0;
   // internal force of the lane's node (both halves for a split node).
   // The wave-uniform trip count selects a straight-line body, so the

@@ -21,6 +21,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
   const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   const int rv = a.solve_rve[sid];
   const RveDev R = a.rves[rv];
+  const double eta_r = a.eta_rve ? a.eta_rve[rv] : a.eta;
   const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
   const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
   const int64_t nb = R.node_off;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
       node_force(s, cur, fint[s]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        double mass = mr[s] * a.eta;
+        double mass = mr[s] * eta_r;
         double fdamp = mass * vr[s][d];
         fr[s][d] = -fint[s][d] - fdamp;
       }
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
         double rn = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          double mass = mr[s] * a.eta;
+          double mass = mr[s] * eta_r;
           double fdamp = mass * vr[s][d];
           double lres = -fint[s][d];
           fr[s][d] = lres - fdamp;
