@@ -214,15 +214,17 @@ int make_plan(mf_batch *b, int prec) {
       if (b->p.block_size && v.block != b->p.block_size) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
       if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) continue;
-      if (regs && p.max_free_deg > 2 * kLaneDegree) continue;
+      if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) continue;
       if ((v.flags & kLinearOnly) && !lin) continue;
       if (regs && b->no_reg_variants) continue;
       size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
       if (L > kMaxLds) continue;
       // register-resident lists first, then smaller thread count, fewer
       // slots, richer flags
-      long key = (regs ? 0L : 1000000000L) + (long)v.block * 1000000 +
-                 (long)v.npt * 1000 - v.flags;
+      // register variants: one node per thread before several, then the
+      // fewest threads
+      long key = regs ? (long)v.npt * 10000000L + v.block
+                      : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
         best = i;
         best_key = key;

@@ -156,6 +156,11 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, 640, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 768, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf},
+    // several free nodes per thread (the spill-free instantiations)
+    {MF_PREC_FP32C, 768, 2, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},
 };
 
 const Variant *variants(int *count) {

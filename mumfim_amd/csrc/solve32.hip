@@ -395,19 +395,21 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 
 // ============================================================================
 // FP32 + compensated kernel with per-lane fiber lists in registers.
-// Free nodes (degree-sorted, descending) map to lanes; a node of degree above
-// kLaneDeg takes two adjacent lanes (even/odd) that each gather half of its
-// fibers and exchange the partial forces with one DPP quad swap.  Every lane
-// therefore holds at most kLaneDeg fibers: the LDS byte offset of the other
-// end, the exactly rounded FP32 reference vector d0 = X_w - X_v and either
-// EA/(2 l0^3) (linear-only RVEs, t = that * s) or 1/l0.  Entries past a
-// lane's count address the node's own row with zero constants and contribute
-// exactly zero, so each wave runs its uniform trip count without masks.  A
-// step reads one neighbour row (FP32 displacement hi) per fiber from LDS.
+// Free nodes (degree-sorted, descending) map to lanes, NPT slots per lane:
+// slot 0 covers ranks [0, B - S), slot s >= 1 ranks [sB - S, (s+1)B - S),
+// where S nodes of degree above kLaneDeg (the first ranks) take two adjacent
+// lanes of slot 0 (even/odd) that each gather half of the fibers and exchange
+// the partial forces with one DPP quad swap.  Every slot therefore holds at
+// most kLaneDeg fibers: the LDS byte offset of the other end, the exactly
+// rounded FP32 reference vector d0 = X_w - X_v and either EA/(2 l0^3)
+// (linear-only RVEs, t = that * s) or 1/l0.  Entries past a slot's count
+// address the node's own row with zero constants and contribute exactly
+// zero, so each wave runs its uniform trip count without masks.  A step
+// reads one neighbour row (FP32 displacement hi) per fiber from LDS.
 // ============================================================================
 constexpr int kLaneDeg = kLaneDegree;
 
-template <int B, bool LIN>
+template <int B, int NPT, bool LIN>
 __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -473,23 +475,26 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     init_node(rb, u);
   }
 
-  // lane -> (node rank, fiber range): the first S free ranks have degree >
-  // kLaneDeg (degree-sorted order) and take lanes 2i, 2i+1.
   const int S = R.n_split;
-  const bool split = tid < 2 * S;
-  const int r = split ? tid >> 1 : tid - S;
-  const bool own = r < Nf;                   // lane computes a node
-  const bool writer = own && (!split || !(tid & 1));  // lane publishes it
-  float uh[3] = {0, 0, 0}, ul[3] = {0, 0, 0}, vr[3] = {0, 0, 0}, ar[3] = {0, 0, 0};
-  float imr = 0.0f;
-  int offs[kLaneDeg];
-  float cf[kLaneDeg], d0x[kLaneDeg], d0y[kLaneDeg], d0z[kLaneDeg];
-  int cnt = 0;
-  {
-    int j0 = 0, dg = 0, base = 0;
-    if (own) {
+  const bool split = tid < 2 * S;  // slot 0 only
+  int rr[NPT];
+  bool own[NPT], writer[NPT];
+  float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
+  int offs[NPT][kLaneDeg];
+  float cf[NPT][kLaneDeg], d0x[NPT][kLaneDeg], d0y[NPT][kLaneDeg], d0z[NPT][kLaneDeg];
+  int Ds[NPT];
+#pragma unroll
+  for (int sl = 0; sl < NPT; ++sl) {
+    const int r = sl == 0 ? (split ? tid >> 1 : tid - S) : sl * B - S + tid;
+    rr[sl] = r;
+    own[sl] = r < Nf;
+    writer[sl] = own[sl] && (sl > 0 || !split || !(tid & 1));
+    imr[sl] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uh[sl][d] = ul[sl][d] = vr[sl][d] = 0.0f;
+    int j0 = 0, dg = 0, base = 0, cnt = 0;
+    if (own[sl]) {
       const int64_t g = nb + r;
-      double u[3];
       double xp[3], up[3];
       for (int d = 0; d < 3; ++d) {
         xp[d] = a.x_in[3 * g + d];
@@ -499,15 +504,14 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         const double inc = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
                            (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
                            (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
-        u[k] = ramp ? up[k] : inc + up[k];
-        uh[k] = (float)u[k];
-        ul[k] = (float)(u[k] - (double)uh[k]);
+        const double u = ramp ? up[k] : inc + up[k];
+        uh[sl][k] = (float)u;
+        ul[sl][k] = (float)(u - (double)uh[sl][k]);
       }
-      imr = (float)(1.0 / a.mass[g]);
+      imr[sl] = (float)(1.0 / a.mass[g]);
       dg = a.deg[g];
-      const int k = r / kWave;
-      base = a.slice_base[R.slice_off + k] + (r & (kWave - 1));
-      if (split) {
+      base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
+      if (sl == 0 && split) {
         const int h = (dg + 1) >> 1;
         j0 = (tid & 1) ? h : 0;
         cnt = (tid & 1) ? dg - h : h;
@@ -515,39 +519,39 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         cnt = dg;
       }
     }
-    const int self = (own ? r : 0) * (int)sizeof(float4);
+    const int self = (own[sl] ? r : 0) * (int)sizeof(float4);
 #pragma unroll
     for (int j = 0; j < kLaneDeg; ++j) {
-      offs[j] = self;
-      cf[j] = 0.0f;
-      d0x[j] = d0y[j] = d0z[j] = 0.0f;
+      offs[sl][j] = self;
+      cf[sl][j] = 0.0f;
+      d0x[sl][j] = d0y[sl][j] = d0z[sl][j] = 0.0f;
       if (j < cnt) {
         const int idx = base + (j0 + j) * kWave;
         const int32_t wv = gw[idx];
         const int w = wv & 0xFFFFFF;
-        offs[j] = w * (int)sizeof(float4) | (LIN ? 0 : (wv & 0xFF000000));
+        offs[sl][j] = w * (int)sizeof(float4) | (LIN ? 0 : (wv & 0xFF000000));
         const double l0 = gl0[idx];
-        cf[j] = LIN ? (float)(EA_half / (l0 * l0 * l0)) : gil[idx];
-        d0x[j] = (float)(X[3 * w] - X[3 * r]);
-        d0y[j] = (float)(X[3 * w + 1] - X[3 * r + 1]);
-        d0z[j] = (float)(X[3 * w + 2] - X[3 * r + 2]);
+        cf[sl][j] = LIN ? (float)(EA_half / (l0 * l0 * l0)) : gil[idx];
+        d0x[sl][j] = (float)(X[3 * w] - X[3 * r]);
+        d0y[sl][j] = (float)(X[3 * w + 1] - X[3 * r + 1]);
+        d0z[sl][j] = (float)(X[3 * w + 2] - X[3 * r + 2]);
       }
     }
-  }
-  // wave-uniform trip count = largest per-lane count in the wave
-  int D = cnt;
+    // wave-uniform trip count = largest per-lane count in the wave
+    int D = cnt;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) D = max(D, __shfl_xor(D, o, kWave));
-  D = __builtin_amdgcn_readfirstlane(D);
+    for (int o = 32; o > 0; o >>= 1) D = max(D, __shfl_xor(D, o, kWave));
+    Ds[sl] = __builtin_amdgcn_readfirstlane(D);
+  }
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
   const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
   int cur = 0;
-  // internal force of the lane's node (both halves for a split node).
-  // The wave-uniform trip count selects a straight-line body, so the
+  // internal force of a slot's node (both halves for a split node).  The
+  // wave-uniform trip count selects a straight-line body, so the
   // neighbour-row loads issue ahead of the arithmetic.
-  auto node_force = [&](int buf, float (&f)[3], bool any_split) {
+  auto node_force = [&](int sl, int buf, float (&f)[3], bool any_split) {
     const unsigned char *P = reinterpret_cast<const unsigned char *>(pos + buf * Np);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](auto KC) {
@@ -555,30 +559,32 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       float4 q[K];
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        q[j] = *reinterpret_cast<const float4 *>(P + (LIN ? offs[j] : (offs[j] & 0xFFFFFF)));
+        q[j] = *reinterpret_cast<const float4 *>(
+            P + (LIN ? offs[sl][j] : (offs[sl][j] & 0xFFFFFF)));
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         // both ends as their FP32-rounded displacements (the low parts only
         // keep the accumulation exact)
-        const float gx = q[j].x - uh[0];
-        const float gy = q[j].y - uh[1];
-        const float gz = q[j].z - uh[2];
-        const float dx = d0x[j] + gx, dy = d0y[j] + gy, dz = d0z[j] + gz;
+        const float gx = q[j].x - uh[sl][0];
+        const float gy = q[j].y - uh[sl][1];
+        const float gz = q[j].z - uh[sl][2];
+        const float dx = d0x[sl][j] + gx, dy = d0y[sl][j] + gy, dz = d0z[sl][j] + gz;
         // l^2 - l0^2 = (d0 + d) . delta
-        const float sl = (d0x[j] + dx) * gx + (d0y[j] + dy) * gy + (d0z[j] + dz) * gz;
+        const float sq = (d0x[sl][j] + dx) * gx + (d0y[sl][j] + dy) * gy +
+                         (d0z[sl][j] + dz) * gz;
         float t;
         if (LIN) {
-          t = cf[j] * sl;
+          t = cf[sl][j] * sq;
         } else {
-          const float il = cf[j];
-          t = react32_t(rxs[(uint32_t)offs[j] >> 24], sl * (il * il), il);
+          const float il = cf[sl][j];
+          t = react32_t(rxs[(uint32_t)offs[sl][j] >> 24], sq * (il * il), il);
         }
         fx -= dx * t;
         fy -= dy * t;
         fz -= dz * t;
       }
     };
-    switch (D) {
+    switch (Ds[sl]) {
 #define MF_DEG_CASE(k) \
   case k:              \
     body(std::integral_constant<int, k>{}); \
@@ -619,52 +625,60 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     }
     return (double)lmin;
   };
+  const bool wave_split = __builtin_amdgcn_readfirstlane(tid & ~(kWave - 1)) < 2 * S;
 
-  float fint[3] = {0, 0, 0};
-  double resid_local = 0.0;
-  if (own) {
-    node_force(cur, fint, true);
+  // The closing half kick of step k and the opening half kick of step k+1
+  // use the same acceleration: v_{k+1/2} = v_{k-1/2} + (h_k + h_{k+1}) a_k,
+  // applied as soon as a_k and h_{k+1} are known (v_{-1/2} = 0, h_{-1} = 0),
+  // so no acceleration is carried across the barrier.
+  const float crit_half = (float)(a.crit * 0.5);
+  float h_prev = 0.0f;
+  float fint[NPT][3];
+  {
+    double resid_local = 0.0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
+    for (int sl = 0; sl < NPT; ++sl)
+      if (own[sl]) node_force(sl, cur, fint[sl], true);
+    double lmin2 = min_length2(cur);
+    block_min_sum<B>(lmin2, resid_local, red);
+    const float h = crit_half * (sqrtf((float)lmin2) * inv_c);
+    const float hk = h_prev + h;
+#pragma unroll
+    for (int sl = 0; sl < NPT; ++sl)
+      if (own[sl]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[sl][d] = vr[sl][d] + hk * (-fint[sl][d] * imr[sl] - eta * vr[sl][d]);
+      }
+    h_prev = h;
   }
-  double lmin2 = min_length2(cur);
-  block_min_sum<B>(lmin2, resid_local, red);
-  float dt_crit = sqrtf((float)lmin2) * inv_c;
 
   int step = 0;
   double residual = 10.0;
   int status = MF_STATUS_OK;
   const int freq = a.freq;
-  const float crit_half = (float)(a.crit * 0.5);
   const double inv_steps = 1.0 / (double)a.n_steps;
-  // The closing half kick of one step and the opening half kick of the next
-  // use the same acceleration: v_{n+1/2} = v_{n-1/2} + (h_{n-1} + h_n) a_n,
-  // with v_{-1/2} = 0 and h_{-1} = 0 (velocity starts at rest).
-  float h_prev = 0.0f;
-  const bool wave_split = __builtin_amdgcn_readfirstlane(tid & ~(kWave - 1)) < 2 * S;
   for (;;) {
-    const float h = crit_half * dt_crit;
-    const float hk = h_prev + h;
-    const float h2 = 2.0f * h;
-    h_prev = h;
+    const float h2 = 2.0f * h_prev;
     const int nxt = cur ^ 1;
     float4 *Pn = pos + nxt * Np;
-    if (own) {
-      float pc[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float vl = vr[d] + hk * ar[d];
-        vr[d] = vl;
-        // u (hi, lo) += 2h v with Fast2Sum: exact when |u_hi| >= |increment|;
-        // otherwise u is within a few increments of zero and the
-        // uncompensated rounding is below eps * |increment|.
-        const float y = ul[d] + h2 * vl;
-        const float sh = uh[d] + y;
-        ul[d] = y - (sh - uh[d]);
-        uh[d] = sh;
-        pc[d] = sh;
+    for (int sl = 0; sl < NPT; ++sl) {
+      if (own[sl]) {
+        float pc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          // u (hi, lo) += 2h v with Fast2Sum: exact when |u_hi| >= |increment|;
+          // otherwise u is within a few increments of zero and the
+          // uncompensated rounding is below eps * |increment|.
+          const float y = ul[sl][d] + h2 * vr[sl][d];
+          const float sh = uh[sl][d] + y;
+          ul[sl][d] = y - (sh - uh[sl][d]);
+          uh[sl][d] = sh;
+          pc[d] = sh;
+        }
+        if (writer[sl]) Pn[rr[sl]] = make_float4(pc[0], pc[1], pc[2], 0.0f);
       }
-      if (writer) Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
     }
     if (ramp) {
       const double alpha = (double)(step + 1) * inv_steps;
@@ -678,19 +692,31 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     cur = nxt;
     const bool refresh = (step % freq) == 0;
     double rsum = 0.0;
-    if (own) {
-      node_force(cur, fint, wave_split);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) ar[d] = -fint[d] * imr - eta * vr[d];
-      if (refresh && writer)
-        rsum = (double)(fint[0] * fint[0] + fint[1] * fint[1] + fint[2] * fint[2]);
+    for (int sl = 0; sl < NPT; ++sl) {
+      if (own[sl]) {
+        node_force(sl, cur, fint[sl], sl == 0 && wave_split);
+        if (refresh && writer[sl])
+          rsum += (double)(fint[sl][0] * fint[sl][0] + fint[sl][1] * fint[sl][1] +
+                           fint[sl][2] * fint[sl][2]);
+      }
     }
+    float h = h_prev;
     if (refresh) {
       double lm = min_length2(cur);
       block_min_sum<B>(lm, rsum, red);
-      dt_crit = sqrtf((float)lm) * inv_c;
+      h = crit_half * (sqrtf((float)lm) * inv_c);
       residual = sqrt(rsum);
     }
+    const float hk = h_prev + h;
+#pragma unroll
+    for (int sl = 0; sl < NPT; ++sl)
+      if (own[sl]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[sl][d] = vr[sl][d] + hk * (-fint[sl][d] * imr[sl] - eta * vr[sl][d]);
+      }
+    h_prev = h;
     ++step;
     if (ramp) {
       if (step >= a.n_steps) break;
@@ -709,7 +735,9 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   float4 *Ph = pos + cur * Np;
   float4 *Pl = pos + (cur ^ 1) * Np;
   __syncthreads();
-  if (writer) Pl[r] = make_float4(ul[0], ul[1], ul[2], 0.0f);
+#pragma unroll
+  for (int sl = 0; sl < NPT; ++sl)
+    if (writer[sl]) Pl[rr[sl]] = make_float4(ul[sl][0], ul[sl][1], ul[sl][2], 0.0f);
   const double alpha_end = ramp ? 1.0 : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
@@ -747,14 +775,14 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
       const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
       float t;
       if (LIN) {
         const double l0 = gl0[idx];
-        t = (float)(EA_half / (l0 * l0 * l0)) * sl;
+        t = (float)(EA_half / (l0 * l0 * l0)) * sq;
       } else {
         const float il = gil[idx];
-        t = react32_t(rxs[(uint32_t)wv >> 24], sl * (il * il), il);
+        t = react32_t(rxs[(uint32_t)wv >> 24], sq * (il * il), il);
       }
       f1 -= dx * t;
       f2 -= dy * t;
@@ -772,14 +800,16 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     if (flags & kWriteU)
       for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
   }
-  if (writer) {
-    const int64_t g = nb + r;
-    for (int d = 0; d < 3; ++d) {
-      const double u = (double)uh[d] + (double)ul[d];
-      if (flags & kWriteX) a.x_out[3 * g + d] = X[3 * r + d] + u;
-      if (flags & kWriteU) a.u_out[3 * g + d] = u;
+#pragma unroll
+  for (int sl = 0; sl < NPT; ++sl)
+    if (writer[sl]) {
+      const int64_t g = nb + rr[sl];
+      for (int d = 0; d < 3; ++d) {
+        const double u = (double)uh[sl][d] + (double)ul[sl][d];
+        if (flags & kWriteX) a.x_out[3 * g + d] = X[3 * rr[sl] + d] + u;
+        if (flags & kWriteU) a.u_out[3 * g + d] = u;
+      }
     }
-  }
   block_sum<B, 6>(st, red);
   if (tid == 0) {
     const double sv = R.scale / a.V[rv];
@@ -812,19 +842,39 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
-#define MF_REG_LAUNCH(B)                                                       \
-  if (v.npt == 1 && v.block == B && (v.flags & kEllInRegs)) {                  \
-    if (v.flags & kLinearOnly)                                                 \
-      hipLaunchKernelGGL((kern::solve32r<B, true>), dim3(n_blocks), dim3(B), lds, s, a); \
-    else                                                                       \
-      hipLaunchKernelGGL((kern::solve32r<B, false>), dim3(n_blocks), dim3(B), lds, s, a); \
+#define MF_REG_LAUNCH(B, N)                                                    \
+  if (v.npt == N && v.block == B && (v.flags & kEllInRegs)) {                  \
+    auto k = (v.flags & kLinearOnly) ? kern::solve32r<B, N, true>              \
+                                     : kern::solve32r<B, N, false>;            \
+    if (lds > 64 * 1024)                                                       \
+      (void)hipFuncSetAttribute((const void *)k,                               \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)kMaxLds);                                 \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                 \
     return hipGetLastError();                                                  \
   }
-  MF_REG_LAUNCH(256)
-  MF_REG_LAUNCH(512)
-  MF_REG_LAUNCH(640)
-  MF_REG_LAUNCH(768)
-  MF_REG_LAUNCH(1024)
+  MF_REG_LAUNCH(256, 1)
+  MF_REG_LAUNCH(512, 1)
+  MF_REG_LAUNCH(640, 1)
+  MF_REG_LAUNCH(768, 1)
+  MF_REG_LAUNCH(1024, 1)
+#undef MF_REG_LAUNCH
+  // several nodes per thread: only the instantiations that stay spill-free
+#define MF_REG_LAUNCH(B, N, LIN)                                               \
+  if (v.npt == N && v.block == B && (v.flags & kEllInRegs) &&                  \
+      ((v.flags & kLinearOnly) != 0) == LIN) {                                 \
+    auto k = kern::solve32r<B, N, LIN>;                                        \
+    if (lds > 64 * 1024)                                                       \
+      (void)hipFuncSetAttribute((const void *)k,                               \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)kMaxLds);                                 \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                 \
+    return hipGetLastError();                                                  \
+  }
+  MF_REG_LAUNCH(768, 2, true)
+  MF_REG_LAUNCH(512, 3, true)
+  MF_REG_LAUNCH(512, 3, false)
+  MF_REG_LAUNCH(512, 4, true)
 #undef MF_REG_LAUNCH
   return hipErrorInvalidValue;
 }

@@ -280,3 +280,37 @@ def test_deterministic_repeat():
         a = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(n_steps=1000, precision=prec))
         b = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(n_steps=1000, precision=prec))
         assert np.array_equal(a.run(F), b.run(F))
+
+
+def test_register_kernel_slots_agree_bitwise():
+    """The register-resident FP32c kernel with one or several free nodes per
+    thread gives bitwise the same state (per-node arithmetic is the same),
+    and stays within the FP32c bar of the oracle."""
+    say("case: register kernel, 1-4 nodes per thread")
+    for nl in (0.0, 0.5):
+        nets = [mf.generate_network(seed=60 + i, n_nodes=n, nonlinear_fraction=nl)
+                for i, n in enumerate([700, 1000])]
+        F = strain_F(6, 2, 0.05)
+        p = ramp_params(n_steps=600, precision=mf.MF_PREC_FP32C,
+                        reaction_policy=mf.MF_REACT_PER_FIBER)
+        g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
+        _, plan = g.plan()
+        ref = g.run(F)
+        assert np.all(plan[:, 1] == 1), plan  # one node per thread by default
+        forced = mf.BatchedFiberRVEAnalysisExplicit(
+            nets, ramp_params(n_steps=600, precision=mf.MF_PREC_FP32C,
+                              reaction_policy=mf.MF_REACT_PER_FIBER, block_size=512))
+        _, fplan = forced.plan()
+        say(f"  nonlinear {nl}: default {plan[:, :2].tolist()} forced {fplan[:, :2].tolist()}")
+        assert np.all(fplan[:, 1] >= 2)
+        # per-node arithmetic is the same: the committed state agrees bitwise;
+        # the FP64 stress sum runs over a different number of threads
+        sf = forced.run(F)
+        ref_x, ref_u, _ = g.get_state()
+        fx, fu, _ = forced.get_state()
+        assert np.array_equal(fx, ref_x) and np.array_equal(fu, ref_u)
+        assert rel_err(sf, ref) <= 1e-13
+        o = Oracle(nets, p)
+        err = rel_err(ref, o.run(F))
+        say(f"  vs oracle {err:.2e}")
+        assert err <= FP32C_TOL
