@@ -52,6 +52,7 @@ extern "C" {
 #define MF_STATUS_NAN 1          /* NaN/Inf in residual or stress            */
 #define MF_STATUS_STEP_CAP 2     /* relax mode hit max_steps before tol      */
 #define MF_STATUS_FALLBACK 3     /* surrogate rejected, explicit solve used  */
+#define MF_STATUS_ENERGY 4       /* energy balance check failed (diagnostics) */
 
 /* ---- fiber reactions (FiberReactions.h:85-123, FiberNetworkIO_impl.h:123-145) */
 #define MF_REACTION_LINEAR 0
@@ -115,13 +116,27 @@ typedef struct mf_solver_params {
   int32_t n_steps;              /* ramp mode: steps per solve                 */
   int32_t max_steps;            /* relax mode: step cap (status STEP_CAP)     */
   int32_t dt_update_freq;       /* steps between dt / residual refresh (100)  */
-  int32_t block_size;           /* threads per solve; 0 = autotuned           */
-  int32_t pad_;
+  int32_t block_size;           /* threads per solve; 0 = planner's choice    */
+  int32_t amplitude;            /* ramp mode: MF_AMP_* boundary amplitude     */
   double visc_damp_coeff;       /* eta, mass-proportional damping             */
   double crit_time_scale_factor;/* dt = factor * min(l) / c (0.8)             */
   double tolerance;             /* relax mode residual tolerance (1e-6)      */
   double fd_step;               /* tangent strain step h                      */
+  /* Energy diagnostics of the non-batched explicit solver
+   * (FiberRVEAnalysisExplicit_impl.h:178-208, 1101-1138), FP64 only:
+   * W_int, W_ext, W_damp, W_kin per solve; with energy_check_eps > 0 a solve
+   * whose |W_kin + W_int + W_damp - W_ext| exceeds eps * max|W| at a check
+   * (every dt_update_freq steps after step 10, and at the end) stops with
+   * status MF_STATUS_ENERGY. */
+  int32_t energy_diagnostics;   /* 1: track energies                           */
+  int32_t pad_;
+  double energy_check_eps;      /* 0: no check                                 */
 } mf_solver_params;
+
+/* Ramp-mode boundary amplitude alpha(xi), xi = k / n_steps. */
+#define MF_AMP_LINEAR 0         /* alpha = xi                                */
+#define MF_AMP_SMOOTHSTEP 1     /* xi^3 (10 - 15 xi + 6 xi^2), SmoothAmp,
+                                   ExplicitAmplitude.h:26-62                 */
 
 typedef struct mf_batch mf_batch;
 
@@ -168,6 +183,9 @@ void *mf_batch_stream(mf_batch *b);
 /* Kernel time (ms, HIP events on the batch stream) of the explicit-solve
  * launches of the last run, and how many launches that was. */
 int mf_batch_last_solve_ms(mf_batch *b, double *ms, int32_t *n_launches);
+/* Energies of the last run() (energy_diagnostics on): n x 4 doubles
+ * W_int, W_ext, W_damp, W_kin per RVE. */
+int mf_batch_last_energies(mf_batch *b, double *W);
 
 /* Orientation tensors of the current configuration. omega: n x 9. */
 int mf_batch_orientation_3d(mf_batch *b, double *omega);

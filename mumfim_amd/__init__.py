@@ -24,6 +24,8 @@ MF_REACT_BATCHED_REFERENCE, MF_REACT_PER_FIBER = 0, 1
 MF_TANGENT_FORWARD, MF_TANGENT_CENTRAL = 0, 1
 MF_REACTION_LINEAR, MF_REACTION_NONLINEAR = 0, 1
 MF_STATUS_OK, MF_STATUS_NAN, MF_STATUS_STEP_CAP, MF_STATUS_FALLBACK = 0, 1, 2, 3
+MF_STATUS_ENERGY = 4
+MF_AMP_LINEAR, MF_AMP_SMOOTHSTEP = 0, 1
 
 
 class mf_reaction(C.Structure):
@@ -46,10 +48,12 @@ class mf_solver_params(C.Structure):
                 ("reaction_policy", C.c_int32), ("tangent_scheme", C.c_int32),
                 ("tangent_precision", C.c_int32), ("n_steps", C.c_int32),
                 ("max_steps", C.c_int32), ("dt_update_freq", C.c_int32),
-                ("block_size", C.c_int32), ("pad_", C.c_int32),
+                ("block_size", C.c_int32), ("amplitude", C.c_int32),
                 ("visc_damp_coeff", C.c_double),
                 ("crit_time_scale_factor", C.c_double),
-                ("tolerance", C.c_double), ("fd_step", C.c_double)]
+                ("tolerance", C.c_double), ("fd_step", C.c_double),
+                ("energy_diagnostics", C.c_int32), ("pad_", C.c_int32),
+                ("energy_check_eps", C.c_double)]
 
 
 class mf_gen_params(C.Structure):
@@ -102,6 +106,7 @@ SIGNATURES = {
     "mf_batch_synchronize": (C.c_int, [_P]),
     "mf_batch_stream": (C.c_void_p, [_P]),
     "mf_batch_last_solve_ms": (C.c_int, [_P, _D, _IP]),
+    "mf_batch_last_energies": (C.c_int, [_P, _D]),
     "mf_batch_orientation_3d": (C.c_int, [_P, _D]),
     "mf_batch_orientation_2d": (C.c_int, [_P, _D, _I, _D]),
     "mf_batch_total_nodes": (C.c_int64, [_P]),
@@ -446,6 +451,12 @@ class BatchedFiberRVEAnalysisExplicit:
         V = np.zeros(self.n)
         check(lib().mf_batch_get_state(self.h, _dptr(x), _dptr(u), _dptr(V)))
         return x, u, V
+
+    def last_energies(self):
+        """W_int, W_ext, W_damp, W_kin of the last run (energy diagnostics)."""
+        W = np.zeros((self.n, 4))
+        check(lib().mf_batch_last_energies(self.h, _dptr(W)))
+        return W
 
     def last_solve_ms(self):
         ms = C.c_double()

@@ -101,6 +101,10 @@ bool params_valid(const mf_solver_params &p) {
          (p.mode != MF_MODE_RAMP || p.n_steps >= 1) && p.fd_step > 0 &&
          p.fd_step < 0.25 && p.crit_time_scale_factor > 0 &&
          p.visc_damp_coeff >= 0 &&
+         (p.amplitude == MF_AMP_LINEAR || p.amplitude == MF_AMP_SMOOTHSTEP) &&
+         (p.energy_diagnostics == 0 ||
+          (p.energy_diagnostics == 1 && p.precision == MF_PREC_FP64)) &&
+         p.energy_check_eps >= 0 &&
          (p.block_size == 0 || p.block_size == 256 || p.block_size == 512 ||
           p.block_size == 640 || p.block_size == 768 || p.block_size == 1024);
 }
@@ -165,6 +169,9 @@ struct mf_batch {
   std::vector<size_t> id_group_lds[2];              // LDS bytes per group
   DevBuf<double> lastF_dev;
   bool lastF_on_device = false;
+  // energies of the last run() (FP64 energy diagnostics), 4 per RVE
+  DevBuf<double> energy;
+  bool have_energy = false;
   // state is the reference configuration (no update run, no set_state)
   bool at_reference = true;
   // per-RVE viscous damping (empty: visc_damp_coeff for all)
@@ -216,6 +223,8 @@ int make_plan(mf_batch *b, int prec) {
       if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) continue;
       if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) continue;
       if ((v.flags & kLinearOnly) && !lin) continue;
+      if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
+        continue;
       if (regs && b->no_reg_variants) continue;
       size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
       if (L > kMaxLds) continue;
@@ -452,6 +461,8 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.freq = b->p.dt_update_freq;
   a.eta = b->p.visc_damp_coeff;
   a.eta_rve = b->eta_rve.p;
+  a.amplitude = b->p.amplitude;
+  a.energy_eps = b->p.energy_check_eps;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
   std::vector<std::pair<int, int>> groups;
@@ -492,6 +503,8 @@ void fill_args(mf_batch *b, SolveArgs &a) {
   a.freq = b->p.dt_update_freq;
   a.eta = b->p.visc_damp_coeff;
   a.eta_rve = b->eta_rve.p;
+  a.amplitude = b->p.amplitude;
+  a.energy_eps = b->p.energy_check_eps;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
 }
@@ -636,6 +649,11 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
   a.solve_flags = b->id_flags[prec].p + (update ? R : 0);
   a.sigma = sigma_dev ? sigma_dev : b->sigma.p;
   a.status = status_dev ? status_dev : b->status.p;
+  b->have_energy = prec == MF_PREC_FP64 && b->p.energy_diagnostics;
+  if (b->have_energy) {
+    MF_HIP(b->energy.ensure(4 * (size_t)R));
+    a.energy = b->energy.p;
+  }
   {
     int e = launch_groups(b, a, b->id_order[prec].p, b->id_groups[prec],
                           b->id_group_lds[prec]);
@@ -872,16 +890,32 @@ int mf_batch_set_params(mf_batch *b, const mf_solver_params *params) {
   }
   const bool mode_changed = params->mode != b->p.mode;
   const int old_bs = b->p.block_size;
+  const int old_en = b->p.energy_diagnostics;
   b->p = *params;
   if (mode_changed) {
     b->have_prev = false;
   }
-  if (old_bs != params->block_size)
+  if (old_bs != params->block_size || old_en != params->energy_diagnostics)
     for (int prec = 0; prec < 2; ++prec) {
       int e = make_plan(b, prec);
       if (e != MF_OK) return e;
     }
   return build_identity_tables(b);
+}
+
+int mf_batch_last_energies(mf_batch *b, double *W) {
+  if (!b || !W) {
+    set_error("mf_batch_last_energies: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  if (!b->have_energy) {
+    set_error("mf_batch_last_energies: the last run had no FP64 energy diagnostics");
+    return MF_ERR_STATE;
+  }
+  MF_HIP(hipMemcpyAsync(W, b->energy.p, sizeof(double) * 4 * b->R, hipMemcpyDeviceToHost,
+                        b->stream));
+  MF_HIP(hipStreamSynchronize(b->stream));
+  return MF_OK;
 }
 
 int mf_batch_set_damping(mf_batch *b, const double *eta) {

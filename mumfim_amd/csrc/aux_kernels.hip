@@ -143,8 +143,12 @@ __global__ __launch_bounds__(B) void orientation_kernel(
 }  // namespace kern
 
 #define MF_VARIANT_ENTRY(P, B, N, F) {P, B, N, F},
+// FP64 with energy diagnostics; the list's FP32c entries become inert
+// placeholders (precision -1 matches no plan)
+#define MF_ENERGY_ENTRY(P, B, N, F) {P == MF_PREC_FP64 ? P : -1, B, N, F | kEnergy},
 static const Variant kVariants[] = {
     MF_VARIANT_LIST(MF_VARIANT_ENTRY)
+    MF_VARIANT_LIST(MF_ENERGY_ENTRY)
     // FP32c with incidence lists in registers (one free node per thread)
     {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
@@ -182,6 +186,7 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
   }
   b += sizeof(double) * NW * kern::kRedStride;
   b += sizeof(double) * 6 * Nbp;
+  if (v.flags & kEnergy) b += sizeof(double) * 3 * Nbp;
   b += (sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15);
   if (v.flags & kEllInLds)
     b += (size_t)ell_free_entries *

@@ -63,6 +63,15 @@ __device__ inline void block_sum(double (&v)[NV], double *red) {
   }
 }
 
+// Ramp-mode boundary amplitude after step k (1-based) of S:
+// linear xi = k/S, or SmoothAmp xi^3 (10 - 15 xi + 6 xi^2)
+// (ExplicitAmplitude.h:26-40), evaluated in the reference's operation order.
+__device__ inline double ramp_alpha(int k, int S, int amplitude) {
+  const double xi = (double)k / (double)S;
+  if (amplitude == MF_AMP_SMOOTHSTEP) return xi * xi * xi * (10 - 15 * xi + 6 * xi * xi);
+  return xi;
+}
+
 // min of a (slot 0) and sum of b (slot 1) in one pass.
 template <int B>
 __device__ inline void block_min_sum(double &a, double &b, double *red) {

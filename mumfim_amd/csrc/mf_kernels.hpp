@@ -40,6 +40,10 @@ struct SolveArgs {
   // per-RVE viscous damping (BatchedFiberRVEAnalysisExplicit.h:213-218);
   // NULL: eta for every RVE
   const double *eta_rve;
+  int32_t amplitude;        // MF_AMP_* (ramp mode)
+  int32_t pad_;
+  double energy_eps;        // > 0: energy balance check (energy variants)
+  double *energy;           // 4 per solve (energy variants), may be NULL
 };
 
 // A kernel variant: threads per solve, free-node slots per thread and the
@@ -49,6 +53,7 @@ enum : int32_t {
   kDoubleBuf = 2,   // positions double-buffered in LDS (one barrier per step)
   kEllInRegs = 4,   // incidence lists in registers, one free node per thread
   kLinearOnly = 8,  // every fiber of the RVE uses one linear reaction
+  kEnergy = 16,     // FP64 with energy diagnostics
 };
 
 

@@ -240,7 +240,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       }
     }
     if (ramp) {
-      const double alpha = (double)(step + 1) / (double)a.n_steps;
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B)
         Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
                                  (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       if (r < Nf) Pl[r] = make_float4(ul[s][0], ul[s][1], ul[s][2], 0.0f);
     }
   }
-  const double alpha_end = ramp ? 1.0 : 0.0;
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
     for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
@@ -657,7 +657,6 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   double residual = 10.0;
   int status = MF_STATUS_OK;
   const int freq = a.freq;
-  const double inv_steps = 1.0 / (double)a.n_steps;
   for (;;) {
     const float h2 = 2.0f * h_prev;
     const int nxt = cur ^ 1;
@@ -681,7 +680,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       }
     }
     if (ramp) {
-      const double alpha = (double)(step + 1) * inv_steps;
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B)
         Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
                                  (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
@@ -738,7 +737,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
 #pragma unroll
   for (int sl = 0; sl < NPT; ++sl)
     if (writer[sl]) Pl[rr[sl]] = make_float4(ul[sl][0], ul[sl][1], ul[sl][2], 0.0f);
-  const double alpha_end = ramp ? 1.0 : 0.0;
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
     for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];

@@ -10,7 +10,14 @@ namespace kern {
 // ============================================================================
 // FP64 kernel
 // ============================================================================
-template <int B, int NPT, int FL>
+// EN adds the energy accounting of the non-batched explicit solver
+// (computeEnergies_ / computeKineticEnergy_ / checkEnergyBalance_,
+// FiberRVEAnalysisExplicit_impl.h:1101-1138, 590-600): per step
+// W_int += 1/2 du.(f_int_prev + f_int) over every node, W_ext the same over
+// the prescribed boundary nodes (their reaction is their internal force),
+// W_damp += 1/2 du.(f_damp_prev + f_damp) with f_damp = eta m v, and
+// W_kin = 1/2 m v.v.
+template <int B, int NPT, int FL, bool EN>
 __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
 #pragma clang fp contract(off)
   constexpr bool kEll = (FL & kEllInLds) != 0;
@@ -34,6 +41,8 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
   off += sizeof(double) * NW * kRedStride;
   double *bst = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * 6 * Nbp;
+  double *fbl = reinterpret_cast<double *>(smem + off);  // boundary f_int, last step
+  off += EN ? sizeof(double) * 3 * Nbp : 0;
   ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
   off += align16(sizeof(ReactDev) * kMaxReactions);
   double *l0s = reinterpret_cast<double *>(smem + off);
@@ -176,6 +185,44 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
     return lmin;
   };
 
+  // internal force of boundary node Nf + b at the positions in buffer `buf`
+  auto bnd_force = [&](int b, int buf, double (&f)[3]) {
+    const double *P = pos + (kDbuf ? buf * 3 * Np : 0);
+    const int r = Nf + b;
+    const double xb0 = P[r], xb1 = P[Np + r], xb2 = P[2 * Np + r];
+    const int base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
+    const int dg = a.deg[nb + r];
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const double l0 = gl0[idx];
+      const int w = wv & 0xFFFFFF;
+      const double dx = P[w] - xb0;
+      const double dy = P[Np + w] - xb1;
+      const double dz = P[2 * Np + w] - xb2;
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      const double t = frc / l;
+      f1 = f1 - dx * t;
+      f2 = f2 - dy * t;
+      f3 = f3 - dz * t;
+    }
+    f[0] = f1;
+    f[1] = f2;
+    f[2] = f3;
+  };
+  // energy accumulators (sums of du.(f_prev + f), halved when reported)
+  double w_int = 0.0, w_ext = 0.0, w_damp = 0.0;
+  double fl[NPT][3], dl[NPT][3], du[NPT][3];
+
   // ---- RunInitialization (TeamOuterLoop.h:215-289) ---------------------------
   double fint[NPT][3];
   double resid_local = 0.0;
@@ -194,6 +241,44 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
   double lmin = min_length(cur);
   block_min_sum<B>(lmin, resid_local, red);
   double dt_crit = lmin / c_sound;
+  if (EN) {
+#pragma unroll
+    for (int s = 0; s < NPT; ++s)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        fl[s][d] = s * B + tid < Nf ? fint[s][d] : 0.0;
+        dl[s][d] = 0.0;
+        du[s][d] = 0.0;
+      }
+    for (int b = tid; b < Nb; b += B) {
+      double f[3];
+      bnd_force(b, cur, f);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fbl[d * Nbp + b] = f[d];
+    }
+  }
+  // totals of the four energies; a failed balance check stops the solve
+  auto energy_totals = [&](double (&W)[4]) {
+    double wk = 0.0;
+#pragma unroll
+    for (int s = 0; s < NPT; ++s)
+      if (s * B + tid < Nf)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) wk += mr[s] * vr[s][d] * vr[s][d];
+    W[0] = w_int;
+    W[1] = w_ext;
+    W[2] = w_damp;
+    W[3] = wk;
+    __syncthreads();  // every wave is done with the previous use of red
+    block_sum<B, 4>(W, red);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) W[i] = 0.5 * W[i];
+  };
+  auto balance_ok = [&](const double (&W)[4]) {
+    const double bal = W[3] + W[0] + W[2] - W[1];
+    const double m = fmax(fmax(fabs(W[1]), fmax(fabs(W[0]), fabs(W[3]))), fabs(W[2]));
+    return fabs(bal) <= a.energy_eps * m;
+  };
 
   // ---- main loop (TeamOuterLoop.h:340-371) -----------------------------------
   int step = 0;
@@ -215,12 +300,13 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
           double vl = vr[s][d] + h * fr[s][d] / mr[s];
           vr[s][d] = vl;
           xr[s][d] = xr[s][d] + 2 * h * vl;
+          if (EN) du[s][d] = 2 * h * vl;
           Pn[d * Np + r] = xr[s][d];
         }
       }
     }
     if (ramp) {
-      const double alpha = (double)(step + 1) / (double)a.n_steps;
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -243,8 +329,32 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
           double lres = -fint[s][d];
           fr[s][d] = lres - fdamp;
           rn += lres * lres;
+          if (EN) {
+            w_int += du[s][d] * (fl[s][d] + fint[s][d]);
+            w_damp += du[s][d] * (dl[s][d] + fdamp);
+            fl[s][d] = fint[s][d];
+            dl[s][d] = fdamp;
+          }
         }
         rsum += rn;
+      }
+    }
+    if (EN && ramp) {
+      // prescribed boundary: du_b from the amplitude, reaction = f_int
+      const double a0 = step == 0 ? 0.0 : ramp_alpha(step, a.n_steps, a.amplitude);
+      const double a1 = ramp_alpha(step + 1, a.n_steps, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+        double f[3];
+        bnd_force(b, cur, f);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double x1 = bst[d * Nbp + b] + a1 * bst[(3 + d) * Nbp + b];
+          const double x0 = bst[d * Nbp + b] + a0 * bst[(3 + d) * Nbp + b];
+          const double t = (x1 - x0) * (fbl[d * Nbp + b] + f[d]);
+          w_int += t;
+          w_ext += t;
+          fbl[d * Nbp + b] = f[d];
+        }
       }
     }
     if (refresh) {
@@ -262,6 +372,14 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
       }
     }
     ++step;
+    if (EN && a.energy_eps > 0.0 && step > 10 && step % freq == 0) {
+      double W[4];
+      energy_totals(W);
+      if (!balance_ok(W)) {
+        status = MF_STATUS_ENERGY;
+        break;
+      }
+    }
     if (ramp) {
       if (step >= a.n_steps) break;
     } else {
@@ -273,6 +391,15 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
     }
   }
   if (residual != residual) status = MF_STATUS_NAN;
+  if (EN) {
+    double W[4];
+    energy_totals(W);
+    if (status == MF_STATUS_OK && a.energy_eps > 0.0 && step > 10 && !balance_ok(W))
+      status = MF_STATUS_ENERGY;
+    if (a.energy && tid == 0)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a.energy[4 * (int64_t)sid + i] = W[i];
+  }
 
   // ---- FinalizeData + computeCauchyStress ------------------------------------
   // Boundary internal forces gather every incident fiber (TeamOuterLoop.h:
@@ -280,7 +407,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
   const double *P = pos + (kDbuf ? cur * 3 * Np : 0);
   __syncthreads();  // the last refresh reduction has been read by every wave
   double st[6] = {0, 0, 0, 0, 0, 0};
-  const double alpha_end = ramp ? 1.0 : 0.0;
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     const int r = Nf + b;
     double xb[3];
@@ -360,8 +487,9 @@ hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
                   size_t lds, hipStream_t s) {
 #define MF_VARIANT_LAUNCH(P, B, N, F)                                        \
   if constexpr (P == MF_PREC_FP64)                                           \
-    if (v.block == B && v.npt == N && v.flags == F) {                       \
-    auto k = kern::solve64<B, N, F>;                                          \
+    if (v.block == B && v.npt == N && (v.flags & ~kEnergy) == F) {          \
+    auto k = (v.flags & kEnergy) ? kern::solve64<B, N, F, true>              \
+                                 : kern::solve64<B, N, F, false>;            \
     if (lds > 64 * 1024)                                                     \
       (void)hipFuncSetAttribute((const void *)k,                             \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, \

@@ -96,6 +96,8 @@ struct ORve {
   std::vector<double> x_prev, u_prev;
   double F_last[9];
   double current_stress[6] = {0, 0, 0, 0, 0, 0};
+  // W_int, W_ext, W_damp, W_kin of the last solve (energy diagnostics)
+  double energy[4] = {0, 0, 0, 0};
 };
 
 struct OBatch {
@@ -178,6 +180,39 @@ double get_forces(ORve &R, int nv, double eta) {
   return sqrt(residual);
 }
 
+// Ramp amplitude after step k of S: linear, or SmoothAmp
+// (ExplicitAmplitude.h:26-40).
+double ramp_alpha(int k, int S, int amplitude) {
+  const double xi = (double)k / (double)S;
+  if (amplitude == MF_AMP_SMOOTHSTEP) return xi * xi * xi * (10 - 15 * xi + 6 * xi * xi);
+  return xi;
+}
+
+// Internal forces of the boundary vertices (indices Nf..N-1) by a scatter
+// over every fiber in fiber order (the getForces scatter, restricted).
+void boundary_forces(const ORve &R, std::vector<double> &fb) {
+  const int Nf = R.N - R.Nb;
+  std::fill(fb.begin(), fb.end(), 0.0);
+  for (int i = 0; i < R.M; ++i) {
+    const int n1 = R.conn[2 * i], n2 = R.conn[2 * i + 1];
+    if (n1 < Nf && n2 < Nf) continue;
+    const double frc = fiber_force(R, i, R.l0[i], R.l[i]);
+    const double frc_ovr_l = frc / R.l[i];
+    for (int k = 0; k < 3; ++k) {
+      const double e = (R.x[3 * n2 + k] - R.x[3 * n1 + k]) * frc_ovr_l;
+      if (n1 >= Nf) fb[3 * (n1 - Nf) + k] -= e;
+      if (n2 >= Nf) fb[3 * (n2 - Nf) + k] += e;
+    }
+  }
+}
+
+// checkEnergyBalance_ (FiberRVEAnalysisExplicit_impl.h:590-600)
+bool energy_balance_ok(const double W[4], double eps) {
+  const double bal = W[3] + W[0] + W[2] - W[1];
+  return fabs(bal) <= eps * std::max(std::max(fabs(W[1]), std::max(fabs(W[0]), fabs(W[3]))),
+                                     fabs(W[2]));
+}
+
 // One solve of one RVE.  Returns steps taken; status in *status.
 int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
               bool update, double sigma[6], int *status) {
@@ -221,6 +256,28 @@ int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
     }
   dt_crit = element_lengths(R, R.x, R.l) / R.c;
   get_forces(R, Nf, p.visc_damp_coeff);
+  // energy diagnostics (computeEnergies_, computeKineticEnergy_,
+  // FiberRVEAnalysisExplicit_impl.h:1101-1138): sums of du.(f_prev + f)
+  const bool en = p.energy_diagnostics != 0;
+  double w_int = 0, w_ext = 0, w_damp = 0;
+  std::vector<double> fl, dl, du, fbl, fbn;
+  if (en) {
+    fl.assign(R.f_int.begin(), R.f_int.begin() + 3 * (size_t)Nf);
+    dl.assign(3 * (size_t)Nf, 0.0);
+    du.assign(3 * (size_t)Nf, 0.0);
+    fbl.assign(3 * (size_t)R.Nb, 0.0);
+    fbn.assign(3 * (size_t)R.Nb, 0.0);
+    boundary_forces(R, fbl);
+  }
+  auto totals = [&](double W[4]) {
+    double wk = 0;
+    for (int i = 0; i < Nf; ++i)
+      for (int j = 0; j < 3; ++j) wk += R.mass[i] * R.v[3 * i + j] * R.v[3 * i + j];
+    W[0] = 0.5 * w_int;
+    W[1] = 0.5 * w_ext;
+    W[2] = 0.5 * w_damp;
+    W[3] = 0.5 * wk;
+  };
   // main loop (TeamOuterLoop.h:340-371)
   int step = 0;
   double residual = 10.0;
@@ -234,10 +291,11 @@ int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
         double v_local = R.v[3 * i + j] + dt * R.f[3 * i + j] / mass;
         R.v[3 * i + j] = v_local;
         R.x[3 * i + j] = R.x[3 * i + j] + 2 * dt * v_local;
+        if (en) du[3 * i + j] = 2 * dt * v_local;
       }
     }
     if (ramp) {
-      double alpha = (double)(step + 1) / (double)p.n_steps;
+      double alpha = ramp_alpha(step + 1, p.n_steps, p.amplitude);
       for (int b = 0; b < R.Nb; ++b)
         for (int k = 0; k < 3; ++k)
           R.x[3 * (Nf + b) + k] = xb0[3 * b + k] + alpha * gb[3 * b + k];
@@ -247,11 +305,46 @@ int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
     if (refresh) dt_crit = lmin / R.c;
     double res = get_forces(R, Nf, p.visc_damp_coeff);
     if (refresh) residual = res;
+    if (en) {
+      for (int i = 0; i < Nf; ++i) {
+        const double mass = R.mass[i] * p.visc_damp_coeff;
+        for (int j = 0; j < 3; ++j) {
+          const int q = 3 * i + j;
+          const double fdamp = mass * R.v[q];
+          w_int += du[q] * (fl[q] + R.f_int[q]);
+          w_damp += du[q] * (dl[q] + fdamp);
+          fl[q] = R.f_int[q];
+          dl[q] = fdamp;
+        }
+      }
+      if (ramp) {
+        const double a0 = step == 0 ? 0.0 : ramp_alpha(step, p.n_steps, p.amplitude);
+        const double a1 = ramp_alpha(step + 1, p.n_steps, p.amplitude);
+        boundary_forces(R, fbn);
+        for (int b = 0; b < R.Nb; ++b)
+          for (int k = 0; k < 3; ++k) {
+            const double x1 = xb0[3 * b + k] + a1 * gb[3 * b + k];
+            const double x0 = xb0[3 * b + k] + a0 * gb[3 * b + k];
+            const double t = (x1 - x0) * (fbl[3 * b + k] + fbn[3 * b + k]);
+            w_int += t;
+            w_ext += t;
+            fbl[3 * b + k] = fbn[3 * b + k];
+          }
+      }
+    }
     for (int i = 0; i < Nf; ++i) {
       double mass = R.mass[i];
       for (int j = 0; j < 3; ++j) R.v[3 * i + j] += dt * R.f[3 * i + j] / mass;
     }
     ++step;
+    if (en && p.energy_check_eps > 0 && step > 10 && step % p.dt_update_freq == 0) {
+      double W[4];
+      totals(W);
+      if (!energy_balance_ok(W, p.energy_check_eps)) {
+        st = MF_STATUS_ENERGY;
+        break;
+      }
+    }
     if (ramp) {
       if (step >= p.n_steps) break;
     } else {
@@ -263,6 +356,12 @@ int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
     }
   } while (true);
   if (residual != residual) st = MF_STATUS_NAN;
+  if (en) {
+    totals(R.energy);
+    if (st == MF_STATUS_OK && p.energy_check_eps > 0 && step > 10 &&
+        !energy_balance_ok(R.energy, p.energy_check_eps))
+      st = MF_STATUS_ENERGY;
+  }
   // FinalizeData (TeamOuterLoop.h:374-434)
   get_forces(R, N, p.visc_damp_coeff);
   for (int i = 0; i < 3 * N; ++i) R.u[i] = R.x[i] - R.X[i];
@@ -399,6 +498,14 @@ int oracle_set_params(oracle_batch *b, const mf_solver_params *p) {
 }
 
 // run(F, sigma, update): one solve per RVE, OpenMP over RVEs.
+double oracle_ramp_alpha(int k, int S, int amplitude) { return ramp_alpha(k, S, amplitude); }
+
+int oracle_last_energies(oracle_batch *b, double *W) {
+  for (size_t r = 0; r < b->rves.size(); ++r)
+    for (int i = 0; i < 4; ++i) W[4 * r + i] = b->rves[r].energy[i];
+  return 0;
+}
+
 int oracle_run(oracle_batch *b, const double *F, int nF, double *sigma,
                int update, int32_t *status, int32_t *steps) {
   const int n = (int)b->rves.size();

@@ -41,6 +41,9 @@ def lib():
         l.oracle_fiber_force.argtypes = [C.POINTER(mf_reaction), C.c_double, C.c_double]
         l.oracle_fiber_force.restype = C.c_double
         l.oracle_nodal_mass.argtypes = [P, C.c_int, D]
+        l.oracle_last_energies.argtypes = [P, D]
+        l.oracle_ramp_alpha.argtypes = [C.c_int, C.c_int, C.c_int]
+        l.oracle_ramp_alpha.restype = C.c_double
         _lib = l
     return _lib
 
@@ -78,6 +81,11 @@ class Oracle:
         self.last_steps = steps
         return (sigma, st) if return_status else sigma
 
+    def last_energies(self):
+        W = np.zeros((self.n, 4))
+        lib().oracle_last_energies(self.h, W.ctypes.data_as(C.POINTER(C.c_double)))
+        return W
+
     def computeMaterialStiffness(self):
         Cm = np.zeros((self.n, 6, 6))
         lib().oracle_tangent(self.h, _dptr(Cm))
@@ -103,6 +111,10 @@ class Oracle:
         m = np.zeros(self.nets[rve].n_nodes)
         lib().oracle_nodal_mass(self.h, rve, _dptr(m))
         return m
+
+
+def ramp_alpha(k, S, amplitude):
+    return lib().oracle_ramp_alpha(k, S, amplitude)
 
 
 def reorder_permutation(nvert, fixed):

@@ -314,3 +314,39 @@ def test_register_kernel_slots_agree_bitwise():
         err = rel_err(ref, o.run(F))
         say(f"  vs oracle {err:.2e}")
         assert err <= FP32C_TOL
+
+
+def test_energy_diagnostics_and_smoothstep_match_oracle(net27):
+    """Energy accounting (FP64 energy variants) and the SmoothAmp ramp against
+    the oracle; the balance check stops the same solves."""
+    say("case: energy diagnostics + smoothstep amplitude")
+    nets = [net27] + [mf.generate_network(seed=70 + i, n_nodes=n, nonlinear_fraction=0.5)
+                      for i, n in enumerate([400, 900])]
+    F = strain_F(7, 3, 0.04)
+    for amp in (mf.MF_AMP_LINEAR, mf.MF_AMP_SMOOTHSTEP):
+        p = ramp_params(n_steps=1500, amplitude=amp, energy_diagnostics=1,
+                        reaction_policy=mf.MF_REACT_PER_FIBER)
+        o = Oracle(nets, p)
+        so = o.run(F)
+        Wo = o.last_energies()
+        g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
+        sg, st = g.run(F, return_status=True)
+        Wg = g.last_energies()
+        es, ew = rel_err(sg, so), rel_err(Wg, Wo)
+        say(f"  amplitude {amp}: sigma {es:.2e} energies {ew:.2e}  W0 {Wg[0].round(12).tolist()}")
+        assert np.all(st == mf.MF_STATUS_OK)
+        assert es <= FP64_TOL and ew <= 1e-10
+        p32 = ramp_params(n_steps=1500, amplitude=amp, precision=mf.MF_PREC_FP32C,
+                          reaction_policy=mf.MF_REACT_PER_FIBER)
+        s32 = mf.BatchedFiberRVEAnalysisExplicit(nets, p32).run(F)
+        assert rel_err(s32, so) <= FP32C_TOL
+    # balance check: a tight eps stops every solve at its first check
+    p = ramp_params(n_steps=1500, energy_diagnostics=1, energy_check_eps=1e-14,
+                    reaction_policy=mf.MF_REACT_PER_FIBER)
+    so, sto = Oracle(nets, p).run(F, return_status=True)
+    sg, stg = mf.BatchedFiberRVEAnalysisExplicit(nets, p).run(F, return_status=True)
+    assert np.array_equal(stg, sto) and np.all(stg == mf.MF_STATUS_ENERGY)
+    assert rel_err(sg, so) <= FP64_TOL
+    with pytest.raises(mf.MfError):
+        mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(energy_diagnostics=1,
+                                                             precision=mf.MF_PREC_FP32C))

@@ -227,3 +227,64 @@ def test_nan_on_degenerate_fiber():
     o = Oracle([net])
     s, st = o.run(np.diag([1.01, 1, 1]), return_status=True)
     assert st[0] == mf.MF_STATUS_NAN
+
+
+# ---------------------------------------------------------------- diagnostics
+def test_smoothstep_amplitude_known_values():
+    """SmoothAmp (ExplicitAmplitude.h:26-40): xi^3 (10 - 15 xi + 6 xi^2)."""
+    from oracle.oracle import ramp_alpha
+    S = 1000
+    assert ramp_alpha(0, S, mf.MF_AMP_SMOOTHSTEP) == 0.0
+    assert ramp_alpha(S // 2, S, mf.MF_AMP_SMOOTHSTEP) == 0.5
+    assert ramp_alpha(S, S, mf.MF_AMP_SMOOTHSTEP) == 1.0
+    assert ramp_alpha(1, S, mf.MF_AMP_SMOOTHSTEP) == pytest.approx(10 / S ** 3, rel=2e-3)
+    for k in (1, 37, 250, 499):  # point symmetry about xi = 1/2
+        s = ramp_alpha(k, S, mf.MF_AMP_SMOOTHSTEP) + ramp_alpha(S - k, S, mf.MF_AMP_SMOOTHSTEP)
+        assert abs(s - 1.0) < 1e-14
+        assert ramp_alpha(k, S, mf.MF_AMP_LINEAR) == k / S
+
+
+def test_smoothstep_ramp_ends_at_the_linear_ramp_state(net27):
+    F = np.diag([1.02, 0.99, 1.0])
+    out = []
+    for amp in (mf.MF_AMP_LINEAR, mf.MF_AMP_SMOOTHSTEP):
+        o = Oracle([net27], mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=20000,
+                                               amplitude=amp))
+        out.append(o.run(F)[0])
+    assert np.abs(out[0] - out[1]).max() <= 2e-3 * np.abs(out[0]).max()
+
+
+def _strain_energy(net, x):
+    """sum over fibers of E A l0 (lambda^2 - 1)^2 / 8: the integral of the
+    linear truss force lambda E A (lambda^2 - 1)/2 over the length."""
+    r = net.reactions[0]
+    EA = r.E * r.radius ** 2 * math.pi
+    a, b = net.fibers[:, 0], net.fibers[:, 1]
+    l0 = np.linalg.norm(net.coords[b] - net.coords[a], axis=1)
+    l = np.linalg.norm(x[b] - x[a], axis=1)
+    lam = l / l0
+    return float(np.sum(EA * l0 * (lam ** 2 - 1) ** 2 / 8))
+
+
+def test_energy_accounting_closed_forms(net27):
+    """W_int tracks the strain energy of the final state, the balance
+    W_kin + W_int + W_damp - W_ext closes, damping dissipates."""
+    F = np.diag([1.03, 1.0, 0.99])
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=4000, energy_diagnostics=1)
+    o = Oracle([net27], p)
+    o.run(F)
+    W_int, W_ext, W_damp, W_kin = o.last_energies()[0]
+    x, _, _ = o.get_state()
+    U = _strain_energy(net27, x)
+    assert W_ext > 0 and W_damp > 0 and W_kin >= 0
+    assert abs(W_int - U) <= 2e-3 * U, (W_int, U)
+    assert abs(W_kin + W_int + W_damp - W_ext) <= 2e-3 * W_ext
+
+
+def test_energy_check_status(net27):
+    F = np.diag([1.03, 1.0, 1.0])
+    for eps, want in ((1e-14, mf.MF_STATUS_ENERGY), (0.5, mf.MF_STATUS_OK)):
+        p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=1000, energy_diagnostics=1,
+                              energy_check_eps=eps)
+        _, st = Oracle([net27], p).run(F, return_status=True)
+        assert st[0] == want
