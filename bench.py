@@ -125,6 +125,15 @@ def bench_cfg3(args):
     log(f"[cfg3] batch built in {time.perf_counter() - t:.1f}s, plan groups {g.plan()[0]}")
     F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
     sig = mf.DeviceArray((R, 6))
+    heur = None
+    if not args.no_tune:
+        wall_h, kms_h = time_solver(g, F_dev, sig, 1, 0)
+        heur = {"ms_per_batch": round(1e3 * wall_h, 3), "launch_groups": g.plan()[0]}
+        t = time.perf_counter()
+        n_timed = g.autotune(mf.MF_PREC_FP32C, steps=args.tune_steps)
+        tune_s = time.perf_counter() - t
+        log(f"[cfg3] autotuned in {tune_s:.1f}s ({n_timed} candidates timed), "
+            f"plan groups {g.plan()[0]}, heuristic plan {heur}")
     wall, kms = time_solver(g, F_dev, sig, args.steps, args.warmup)
     s32 = sig.numpy()
     st = g.stats()
@@ -144,6 +153,9 @@ def bench_cfg3(args):
            "kernel_ms": round(float(np.mean(kms)), 3), "launch_groups": g.plan()[0],
            "rel_stress_err_vs_fp64_subset": {"n": nsub, "max": float(e.max()),
                                               "median": float(np.median(e))}}
+    if heur is not None:
+        res["autotune"] = {"s": round(tune_s, 2), "candidates_timed": n_timed,
+                           "steps_per_candidate": args.tune_steps, "heuristic_plan": heur}
     if not args.no_cpu:
         o = Oracle(nets[:32], mf.default_params(precision=mf.MF_PREC_FP64, **base))
         so = o.run(F[sub[:32]], update_coords=False)
@@ -366,6 +378,9 @@ def main():
     ap.add_argument("--surrogate-networks", type=int, default=65536,
                     help="cfg5: networks in the training data (16 strains each)")
     ap.add_argument("--surrogate-epochs", type=int, default=30)
+    ap.add_argument("--no-tune", action="store_true", help="cfg3: keep the heuristic plan")
+    ap.add_argument("--tune-steps", type=int, default=100,
+                    help="cfg3: ramp steps per autotuner candidate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "

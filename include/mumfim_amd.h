@@ -183,6 +183,24 @@ void *mf_batch_stream(mf_batch *b);
 /* Kernel time (ms, HIP events on the batch stream) of the explicit-solve
  * launches of the last run, and how many launches that was. */
 int mf_batch_last_solve_ms(mf_batch *b, double *ms, int32_t *n_launches);
+/* ---- batching layer (no reference counterpart) -------------------------- */
+/* Autotuner: RVEs are bucketed by reactions, register eligibility, lanes
+ * needed (wavefronts) and node count (units of 256); for every bucket each
+ * kernel variant that fits (threads per solve, nodes per thread, incidence
+ * lists in registers / LDS / HBM, one or two position buffers) is timed on
+ * ramp solves of `steps` steps of the bucket's RVEs, and the fastest is kept
+ * for that bucket.  With cache_path, buckets found in the file are taken
+ * from it and newly timed ones are appended (text: prec lin regok lanes/64
+ * nodes/256 block npt flags us_per_solve).  n_timed (may be NULL) counts
+ * the timed candidates.  The batch state is left untouched. */
+int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps,
+                      const char *cache_path, int32_t *n_timed);
+/* hipGraph mode for mf_batch_run_device_async: the launch sequence of a run
+ * (volume update, F broadcast, every launch group on the batch and auxiliary
+ * streams, state commits) is captured once per (buffers, flags, state
+ * parity) and replayed afterwards. */
+int mf_batch_set_graph(mf_batch *b, int32_t enable);
+
 /* Energies of the last run() (energy_diagnostics on): n x 4 doubles
  * W_int, W_ext, W_damp, W_kin per RVE. */
 int mf_batch_last_energies(mf_batch *b, double *W);

@@ -107,6 +107,8 @@ SIGNATURES = {
     "mf_batch_stream": (C.c_void_p, [_P]),
     "mf_batch_last_solve_ms": (C.c_int, [_P, _D, _IP]),
     "mf_batch_last_energies": (C.c_int, [_P, _D]),
+    "mf_batch_autotune": (C.c_int, [_P, _I, _I, C.c_char_p, _IP]),
+    "mf_batch_set_graph": (C.c_int, [_P, _I]),
     "mf_batch_orientation_3d": (C.c_int, [_P, _D]),
     "mf_batch_orientation_2d": (C.c_int, [_P, _D, _I, _D]),
     "mf_batch_total_nodes": (C.c_int64, [_P]),
@@ -451,6 +453,19 @@ class BatchedFiberRVEAnalysisExplicit:
         V = np.zeros(self.n)
         check(lib().mf_batch_get_state(self.h, _dptr(x), _dptr(u), _dptr(V)))
         return x, u, V
+
+    def autotune(self, precision=None, steps=200, cache_path=None):
+        """Times the fitting kernel variants per RVE bucket and keeps the
+        fastest; returns the number of timed candidates."""
+        prec = self.params.precision if precision is None else precision
+        n = C.c_int32()
+        check(lib().mf_batch_autotune(self.h, int(prec), int(steps),
+                                      None if cache_path is None else str(cache_path).encode(),
+                                      C.byref(n)))
+        return n.value
+
+    def set_graph(self, enable=True):
+        check(lib().mf_batch_set_graph(self.h, int(bool(enable))))
 
     def last_energies(self):
         """W_int, W_ext, W_damp, W_kin of the last run (energy diagnostics)."""

@@ -6,6 +6,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <tuple>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -127,6 +131,15 @@ void parallel_for(int n, F f) {
 
 }  // namespace
 
+// Autotuner bucket (see tune_key).
+struct TuneKey {
+  int prec, lin, regok, lanes64, n256;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(prec, lin, regok, lanes64, n256) <
+           std::tie(o.prec, o.lin, o.regok, o.lanes64, o.n256);
+  }
+};
+
 struct mf_batch {
   mf_solver_params p{};
   int R = 0;
@@ -161,6 +174,28 @@ struct mf_batch {
   std::vector<int> plan[2];
   std::vector<size_t> lds[2];
   int no_reg_variants = 0;  // planner restricted to the LDS/global-list kernels
+  std::map<TuneKey, int> tuned;  // autotuned variant per bucket
+  int plan_version = 0;          // bumped whenever the launch tables change
+  // hipGraph replay of run_device_fast (mf_batch_set_graph)
+  struct Graph {
+    const double *F;
+    int nF, update;
+    double *sigma;
+    int32_t *status;
+    int cur, prec, version;
+    const void *scratch[5];  // library buffers the captured launches use
+    hipGraphExec_t exec;
+  };
+  bool use_graph = false;
+  std::vector<Graph> graphs;
+  hipEvent_t ev_fork = nullptr;  // fork event of captured launches
+  bool capturing = false;
+  void drop_graphs() {
+    if (graphs.empty()) return;
+    (void)hipStreamSynchronize(stream);  // no replay still in flight
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   // one-solve-per-RVE tables kept on the device: RVE ids, launch order per
   // precision, flags per (update) and the broadcast/gathered F buffer
   DevBuf<int32_t> id_rve, id_order[2], id_flags[2];
@@ -181,6 +216,8 @@ struct mf_batch {
   DevBuf<double> sur_sigma, sur_C;
   DevBuf<int32_t> sur_flag;
   ~mf_batch() {
+    drop_graphs();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (int i = 0; i < kAux; ++i) {
@@ -193,9 +230,45 @@ struct mf_batch {
 
 namespace {
 
-// Picks the kernel variant of every RVE for one precision: the fewest
-// threads that cover the free nodes with at most 4 slots per thread, then
-// incidence lists in LDS and double-buffered positions where they fit.
+// Every fiber of the RVE uses a linear reaction with the E*A of reaction 0
+// (the linear-only kernels take EA from reaction 0).
+bool all_linear(const PackedRVE &p) {
+  if (p.reactions.empty() || p.reactions[0].type != MF_REACTION_LINEAR) return false;
+  const double ea0 = p.reactions[0].E * p.reactions[0].area();
+  for (int e = 0; e < p.M; ++e) {
+    const Reaction &q = p.reactions[p.rid[e]];
+    if (q.type != MF_REACTION_LINEAR || q.E * q.area() != ea0) return false;
+  }
+  return true;
+}
+
+// Whether variant v can run RVE p (threads, registers, LDS, reactions,
+// energy diagnostics) under the batch parameters.
+bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE &p,
+                  bool lin) {
+  if (v.prec != prec) return false;
+  if (b->p.block_size && v.block != b->p.block_size) return false;
+  const bool regs = (v.flags & kEllInRegs) != 0;
+  if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) return false;
+  if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) return false;
+  if ((v.flags & kLinearOnly) && !lin) return false;
+  if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
+    return false;
+  if (regs && b->no_reg_variants) return false;
+  return lds_bytes(v, p.N, p.Nb, p.ell_free_entries) <= kMaxLds;
+}
+
+// Autotuner bucket of an RVE: reactions, register eligibility, lanes needed
+// (in wavefronts) and node count (in units of 256).
+TuneKey tune_key(int prec, const PackedRVE &p, bool lin) {
+  return TuneKey{prec, lin ? 1 : 0, p.max_free_deg <= 2 * kLaneDegree ? 1 : 0,
+                 (p.Nf + p.n_split + 63) / 64, (p.N + 255) / 256};
+}
+
+// Picks the kernel variant of every RVE for one precision: the autotuned
+// variant of the RVE's bucket when there is one, otherwise register-resident
+// lists first (one node per thread before several, fewest threads), then the
+// LDS/global-list kernels with the fewest threads and slots.
 int make_plan(mf_batch *b, int prec) {
   int nv = 0;
   const Variant *vs = variants(&nv);
@@ -203,35 +276,17 @@ int make_plan(mf_batch *b, int prec) {
   b->lds[prec].assign(b->R, 0);
   for (int r = 0; r < b->R; ++r) {
     const PackedRVE &p = b->host[r];
+    const bool lin = all_linear(p);
     int best = -1;
+    auto t = b->tuned.find(tune_key(prec, p, lin));
+    if (t != b->tuned.end() && !b->p.block_size && variant_fits(b, prec, vs[t->second], p, lin))
+      best = t->second;
+    const bool use_tuned = best >= 0;
     long best_key = 0;
-    // linear-only kernels take EA from reaction 0: every fiber must use a
-    // linear reaction with that same E*A
-    bool lin = !p.reactions.empty() && p.reactions[0].type == MF_REACTION_LINEAR;
-    if (lin) {
-      const double ea0 = p.reactions[0].E * p.reactions[0].area();
-      for (int e = 0; e < p.M && lin; ++e) {
-        const Reaction &q = p.reactions[p.rid[e]];
-        lin = q.type == MF_REACTION_LINEAR && q.E * q.area() == ea0;
-      }
-    }
-    for (int i = 0; i < nv; ++i) {
+    for (int i = 0; !use_tuned && i < nv; ++i) {
       const Variant &v = vs[i];
-      if (v.prec != prec) continue;
-      if (b->p.block_size && v.block != b->p.block_size) continue;
+      if (!variant_fits(b, prec, v, p, lin)) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
-      if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) continue;
-      if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) continue;
-      if ((v.flags & kLinearOnly) && !lin) continue;
-      if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
-        continue;
-      if (regs && b->no_reg_variants) continue;
-      size_t L = lds_bytes(v, p.N, p.Nb, p.ell_free_entries);
-      if (L > kMaxLds) continue;
-      // register-resident lists first, then smaller thread count, fewer
-      // slots, richer flags
-      // register variants: one node per thread before several, then the
-      // fewest threads
       long key = regs ? (long)v.npt * 10000000L + v.block
                       : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
@@ -382,6 +437,7 @@ int create_from_networks(std::vector<Network> &nets, const mf_solver_params *par
   MF_HIP(hipStreamCreate(&b->stream));
   MF_HIP(hipEventCreate(&b->ev0));
   MF_HIP(hipEventCreate(&b->ev1));
+  MF_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
   for (int i = 0; i < mf_batch::kAux; ++i) {
     MF_HIP(hipStreamCreate(&b->aux[i]));
     MF_HIP(hipEventCreateWithFlags(&b->aux_done[i], hipEventDisableTiming));
@@ -512,6 +568,8 @@ void fill_args(mf_batch *b, SolveArgs &a) {
 // Device-resident tables for the common case of one solve per RVE.
 int build_identity_tables(mf_batch *b) {
   const int R = b->R;
+  b->plan_version++;
+  b->drop_graphs();
   hipStream_t s = b->stream;
   std::vector<int32_t> ids(R);
   for (int r = 0; r < R; ++r) ids[r] = r;
@@ -561,7 +619,10 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
   int nv = 0;
   const Variant *vs = variants(&nv);
   hipStream_t s = b->stream;
-  MF_HIP(hipEventRecord(b->ev0, s));
+  // under capture the fork uses its own event and the timing events stay
+  // outside the graph
+  hipEvent_t fork = b->capturing ? b->ev_fork : b->ev0;
+  MF_HIP(hipEventRecord(fork, s));
   bool used[mf_batch::kAux] = {false, false, false};
   int off = 0;
   for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -569,7 +630,7 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
     if (gi > 0) {
       const int k = (int)((gi - 1) % mf_batch::kAux);
       st = b->aux[k];
-      if (!used[k]) MF_HIP(hipStreamWaitEvent(st, b->ev0, 0));
+      if (!used[k]) MF_HIP(hipStreamWaitEvent(st, fork, 0));
       used[k] = true;
     }
     a.order = order + off;
@@ -585,7 +646,7 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
       MF_HIP(hipEventRecord(b->aux_done[k], b->aux[k]));
       MF_HIP(hipStreamWaitEvent(s, b->aux_done[k], 0));
     }
-  MF_HIP(hipEventRecord(b->ev1, s));
+  if (!b->capturing) MF_HIP(hipEventRecord(b->ev1, s));
   b->last_launches = (int)groups.size();
   return MF_OK;
 }
@@ -671,6 +732,80 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
       MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));
       MF_HIP(hipMemcpyAsync(b->lastF_dev.p, Fs, sizeof(double) * 9 * R,
                             hipMemcpyDeviceToDevice, s));
+      b->lastF_on_device = true;
+    }
+  }
+  return MF_OK;
+}
+
+// run_device_fast through a captured hipGraph: the first call with a given
+// (buffers, flags, state parity, launch tables) captures the launch sequence
+// (volume update, F broadcast, all launch groups on the batch and auxiliary
+// streams, commits), later calls replay it.  Host bookkeeping runs on every
+// call; device buffers are allocated before capture.
+int run_device_graph(mf_batch *b, const double *F_dev, int nF, int update, double *sigma_dev,
+                     int32_t *status_dev) {
+  const int R = b->R;
+  hipStream_t s = b->stream;
+  const int prec = b->p.precision;
+  const bool ramp = b->p.mode == MF_MODE_RAMP;
+  MF_HIP(b->sigma.ensure(6 * (size_t)R));
+  MF_HIP(b->status.ensure(R));
+  if (prec == MF_PREC_FP64 && b->p.energy_diagnostics) MF_HIP(b->energy.ensure(4 * (size_t)R));
+  if (update && ramp) MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));
+  const void *scratch[5] = {b->sigma.p, b->status.p, b->energy.p, b->lastF_dev.p,
+                            b->eta_rve.p};
+  mf_batch::Graph *g = nullptr;
+  for (auto &x : b->graphs)
+    if (x.F == F_dev && x.nF == nF && x.update == update && x.sigma == sigma_dev &&
+        x.status == status_dev && x.cur == b->cur && x.prec == prec &&
+        x.version == b->plan_version && std::equal(scratch, scratch + 5, x.scratch))
+      g = &x;
+  if (!g) {
+    if (b->graphs.size() >= 8) b->drop_graphs();
+    // capture with the state as it is now; the bookkeeping is redone below
+    const int cur0 = b->cur;
+    const bool hr = b->have_run, hp = b->have_prev, ar = b->at_reference,
+               lf = b->lastF_on_device, he = b->have_energy;
+    hipGraph_t graph = nullptr;
+    MF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    b->capturing = true;
+    int e = run_device_fast(b, F_dev, nF, update, sigma_dev, status_dev);
+    b->capturing = false;
+    hipError_t he2 = hipStreamEndCapture(s, &graph);
+    b->cur = cur0;
+    b->have_run = hr;
+    b->have_prev = hp;
+    b->at_reference = ar;
+    b->lastF_on_device = lf;
+    b->have_energy = he;
+    if (e != MF_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return e;
+    }
+    MF_HIP(he2);
+    hipGraphExec_t exec = nullptr;
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MF_HIP(ie);
+    mf_batch::Graph ng{F_dev, nF, update, sigma_dev, status_dev, b->cur, prec, b->plan_version,
+                       {}, exec};
+    std::copy(scratch, scratch + 5, ng.scratch);
+    b->graphs.push_back(ng);
+    g = &b->graphs.back();
+  }
+  MF_HIP(hipEventRecord(b->ev0, s));
+  MF_HIP(hipGraphLaunch(g->exec, s));
+  MF_HIP(hipEventRecord(b->ev1, s));
+  // host bookkeeping of run_device_fast
+  b->have_energy = prec == MF_PREC_FP64 && b->p.energy_diagnostics;
+  if (!ramp) b->at_reference = false;
+  if (update) {
+    b->have_run = true;
+    b->at_reference = false;
+    if (ramp) {
+      b->cur ^= 1;
+      b->have_prev = true;
       b->lastF_on_device = true;
     }
   }
@@ -903,6 +1038,143 @@ int mf_batch_set_params(mf_batch *b, const mf_solver_params *params) {
   return build_identity_tables(b);
 }
 
+int mf_batch_set_graph(mf_batch *b, int32_t enable) {
+  if (!b) return MF_ERR_INVALID;
+  if (!enable) b->drop_graphs();
+  b->use_graph = enable != 0;
+  return MF_OK;
+}
+
+int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char *cache_path,
+                      int32_t *n_timed) {
+  if (!b || (precision != MF_PREC_FP64 && precision != MF_PREC_FP32C) || steps < 1 ||
+      steps > 1000000) {
+    set_error("mf_batch_autotune: bad arguments");
+    return MF_ERR_INVALID;
+  }
+  if (b->p.block_size) {
+    set_error("mf_batch_autotune: block_size forces the variant; set it to 0");
+    return MF_ERR_INVALID;
+  }
+  const int prec = precision;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  std::map<TuneKey, std::vector<int>> buckets;
+  std::vector<char> lin(b->R);
+  for (int r = 0; r < b->R; ++r) {
+    lin[r] = all_linear(b->host[r]);
+    buckets[tune_key(prec, b->host[r], lin[r])].push_back(r);
+  }
+  // cache lines: prec lin regok lanes64 n256 block npt flags us_per_solve
+  std::map<TuneKey, int> cached;
+  if (cache_path) {
+    std::ifstream f(cache_path);
+    std::string line;
+    while (std::getline(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream is(line);
+      TuneKey k;
+      int blk, npt, fl;
+      double us;
+      if (!(is >> k.prec >> k.lin >> k.regok >> k.lanes64 >> k.n256 >> blk >> npt >> fl >> us))
+        continue;
+      for (int i = 0; i < nv; ++i)
+        if (vs[i].prec == k.prec && vs[i].block == blk && vs[i].npt == npt && vs[i].flags == fl)
+          cached[k] = i;
+    }
+  }
+  const mf_solver_params p0 = b->p;
+  const std::vector<int> plan0 = b->plan[prec];
+  const std::vector<size_t> lds0 = b->lds[prec];
+  b->p.mode = MF_MODE_RAMP;
+  b->p.n_steps = steps;
+  int timed = 0;
+  std::ostringstream added;
+  int err = MF_OK;
+  for (const auto &kv : buckets) {
+    const TuneKey &key = kv.first;
+    const std::vector<int> &L = kv.second;
+    auto fits_all = [&](int i) {
+      for (int r : L)
+        if (!variant_fits(b, prec, vs[i], b->host[r], lin[r])) return false;
+      return true;
+    };
+    auto c = cached.find(key);
+    if (c != cached.end() && fits_all(c->second)) {
+      b->tuned[key] = c->second;
+      continue;
+    }
+    std::vector<int> cand;
+    for (int i = 0; i < nv; ++i)
+      if (fits_all(i)) cand.push_back(i);
+    if (cand.empty()) continue;
+    int best = cand[0];
+    double best_us = 0.0;
+    if (cand.size() > 1) {
+      // enough solves to fill the GPU a few times over
+      const int ns = std::max(512, std::min((int)L.size(), 2048));
+      SolveSet S;
+      S.prec = prec;
+      S.rve.resize(ns);
+      S.flags.assign(ns, 0);
+      S.F.assign(9 * (size_t)ns, 0.0);
+      for (int i = 0; i < ns; ++i) {
+        S.rve[i] = L[i % L.size()];
+        S.F[9 * i] = 1.01;
+        S.F[9 * i + 4] = 1.0;
+        S.F[9 * i + 8] = 1.0;
+      }
+      best = -1;
+      for (int i : cand) {
+        for (int r : L) {
+          b->plan[prec][r] = i;
+          b->lds[prec][r] = lds_bytes(vs[i], b->host[r].N, b->host[r].Nb,
+                                      b->host[r].ell_free_entries);
+        }
+        const int cidx = b->cur;
+        double us = 0.0;
+        for (int rep = 0; rep < 2 && err == MF_OK; ++rep) {  // warm-up, then timed
+          err = launch_set(b, S, b->xs[cidx].p, b->us[cidx].p, b->xs[cidx].p, b->us[cidx].p);
+          if (err == MF_OK) err = finish_timing(b);
+          us = 1e3 * b->last_ms / ns;
+        }
+        if (err != MF_OK) break;
+        ++timed;
+        if (best < 0 || us < best_us) {
+          best = i;
+          best_us = us;
+        }
+      }
+      if (err != MF_OK) break;
+    }
+    b->tuned[key] = best;
+    added << key.prec << ' ' << key.lin << ' ' << key.regok << ' ' << key.lanes64 << ' '
+          << key.n256 << ' ' << vs[best].block << ' ' << vs[best].npt << ' ' << vs[best].flags
+          << ' ' << best_us << '\n';
+  }
+  b->p = p0;
+  b->plan[prec] = plan0;
+  b->lds[prec] = lds0;
+  if (err != MF_OK) return err;
+  int e = make_plan(b, prec);
+  if (e != MF_OK) return e;
+  if ((e = build_identity_tables(b)) != MF_OK) return e;
+  if (cache_path && !added.str().empty()) {
+    const bool fresh = !std::ifstream(cache_path).good();
+    std::ofstream f(cache_path, std::ios::app);
+    if (!f) {
+      set_error(std::string("mf_batch_autotune: cannot write ") + cache_path);
+      return MF_ERR_IO;
+    }
+    if (fresh)
+      f << "# mumfim_amd autotune cache: prec lin regok lanes/64 nodes/256 block npt flags "
+           "us_per_solve (gfx950)\n";
+    f << added.str();
+  }
+  if (n_timed) *n_timed = timed;
+  return MF_OK;
+}
+
 int mf_batch_last_energies(mf_batch *b, double *W) {
   if (!b || !W) {
     set_error("mf_batch_last_energies: NULL argument");
@@ -924,6 +1196,7 @@ int mf_batch_set_damping(mf_batch *b, const double *eta) {
     return MF_ERR_INVALID;
   }
   if (!eta) {
+    b->drop_graphs();
     b->eta_rve.release();
     return MF_OK;
   }
@@ -933,6 +1206,7 @@ int mf_batch_set_damping(mf_batch *b, const double *eta) {
       return MF_ERR_INVALID;
     }
   std::vector<double> v(eta, eta + b->R);
+  b->drop_graphs();
   MF_HIP(b->eta_rve.upload(v, b->stream));
   MF_HIP(hipStreamSynchronize(b->stream));
   return MF_OK;
@@ -978,7 +1252,8 @@ int mf_batch_run_device_async(mf_batch *b, const double *F_dev, int32_t nF,
   }
   int e = check_F(b, nF, update_coords);
   if (e != MF_OK) return e;
-  return run_device_fast(b, F_dev, nF, update_coords, sigma_dev, status_dev);
+  if (!b->use_graph) return run_device_fast(b, F_dev, nF, update_coords, sigma_dev, status_dev);
+  return run_device_graph(b, F_dev, nF, update_coords, sigma_dev, status_dev);
 }
 
 int mf_batch_synchronize(mf_batch *b) {

@@ -1,0 +1,112 @@
+"""Batching layer: the autotuner (per-bucket variant timing with a persisted
+cache) and hipGraph replay of the run launch sequence.  Tuned plans keep
+parity (FP64 bitwise with the heuristic plan, FP32c within its bar of the
+oracle); graph replays are bitwise equal to eager runs, with eager work
+between the replays."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mumfim_amd as mf
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def strain_F(seed, n, amp):
+    u = mf.uniform(seed, 77, 9 * n).reshape(n, 3, 3)
+    U = (2 * u - 1) * amp
+    return np.eye(3)[None] + 0.5 * (U + U.transpose(0, 2, 1))
+
+
+def rel_err(a, b):
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    return float((np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-300)).max())
+
+
+def uneven_nets():
+    sizes = [200, 260, 430, 700, 910, 1300, 1650, 2700, 4000]
+    return [mf.generate_network(seed=90 + i, n_nodes=n, nonlinear_fraction=0.5)
+            for i, n in enumerate(sizes)]
+
+
+def test_autotune_keeps_parity_and_uses_the_cache(tmp_path):
+    nets = uneven_nets()
+    F = strain_F(8, len(nets), 0.05)
+    cache = str(tmp_path / "tune.txt")
+    base = dict(mode=mf.MF_MODE_RAMP, n_steps=800, reaction_policy=mf.MF_REACT_PER_FIBER)
+    # FP64: every variant computes the same bits
+    p64 = mf.default_params(precision=mf.MF_PREC_FP64, **base)
+    g = mf.BatchedFiberRVEAnalysisExplicit(nets, p64)
+    ref = g.run(F, update_coords=False)
+    say("case: autotune FP64")
+    nt = g.autotune(mf.MF_PREC_FP64, steps=100, cache_path=cache)
+    say(f"  timed {nt}, plan {g.plan()[1][:, :3].tolist()}")
+    assert nt > 0
+    assert np.array_equal(g.run(F, update_coords=False), ref)
+    # FP32c: within the bar of the oracle; a second batch reads the cache
+    say("case: autotune FP32c")
+    p32 = mf.default_params(precision=mf.MF_PREC_FP32C, **base)
+    g32 = mf.BatchedFiberRVEAnalysisExplicit(nets, p32)
+    nt32 = g32.autotune(steps=100, cache_path=cache)
+    plan32 = g32.plan()[1]
+    s32 = g32.run(F, update_coords=False)
+    so = Oracle(nets, p64).run(F, update_coords=False)
+    err = rel_err(s32, so)
+    say(f"  timed {nt32}, plan {plan32[:, :3].tolist()}, vs oracle {err:.2e}")
+    assert nt32 > 0 and err <= 1e-4
+    again = mf.BatchedFiberRVEAnalysisExplicit(nets, p32)
+    assert again.autotune(steps=100, cache_path=cache) == 0
+    assert np.array_equal(again.plan()[1], plan32)
+    assert np.array_equal(again.run(F, update_coords=False), s32)
+    with open(cache) as f:
+        lines = [l for l in f if not l.startswith("#")]
+    assert len(lines) >= 2
+
+
+@pytest.mark.parametrize("prec,mode", [(mf.MF_PREC_FP32C, mf.MF_MODE_RAMP),
+                                       (mf.MF_PREC_FP64, mf.MF_MODE_RAMP),
+                                       (mf.MF_PREC_FP64, mf.MF_MODE_RELAX)])
+def test_graph_replay_matches_eager(prec, mode):
+    say(f"case: hipGraph replay prec={prec} mode={mode}")
+    g = mf.default_gen_params(seed=300, n_nodes=300, nonlinear_fraction=0.3)
+    n = 48 if mode == mf.MF_MODE_RAMP else 6
+    p = mf.default_params(mode=mode, n_steps=400, precision=prec,
+                          reaction_policy=mf.MF_REACT_PER_FIBER)
+    eager = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(g, n))
+    graph = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(g, n))
+    graph.set_graph(True)
+    amp = 0.03 if mode == mf.MF_MODE_RAMP else 0.003
+    Fs = [strain_F(400 + k, n, amp) for k in range(3)]
+    F_dev = [mf.DeviceArray.from_host(F.reshape(n, 9)) for F in Fs]
+    se, sg = mf.DeviceArray((n, 6)), mf.DeviceArray((n, 6))
+    for rnd in range(2):
+        for k in range(3):  # trial runs: replays of one graph per F buffer
+            eager.run_device(F_dev[k], se)
+            eager.synchronize()
+            graph.run_device(F_dev[k], sg)
+            graph.synchronize()
+            assert np.array_equal(sg.numpy(), se.numpy()), (rnd, k)
+            # eager work on the graph batch between replays
+            assert np.array_equal(graph.run(Fs[k], update_coords=False),
+                                  eager.run(Fs[k], update_coords=False))
+    for k in range(4):  # update runs alternate the state buffers
+        Fu = F_dev[k % 3]
+        eager.run_device(Fu, se, update_coords=True)
+        eager.synchronize()
+        graph.run_device(Fu, sg, update_coords=True)
+        graph.synchronize()
+        assert np.array_equal(sg.numpy(), se.numpy()), k
+        xe, ue, Ve = eager.get_state()
+        xg, ug, Vg = graph.get_state()
+        assert np.array_equal(xg, xe) and np.array_equal(ug, ue) and np.array_equal(Vg, Ve)
+    ms, nl = graph.last_solve_ms()
+    assert ms > 0 and nl >= 1
