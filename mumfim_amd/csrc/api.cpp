@@ -242,6 +242,21 @@ bool all_linear(const PackedRVE &p) {
   return true;
 }
 
+// At most one linear and one nonlinear reaction in the table, and a toe
+// range whose expm1 argument stays within the polynomial's bound (the FP32c
+// register kernel's one-linear-one-nonlinear mode).
+bool mix2_ok(const PackedRVE &p) {
+  if (p.reactions.empty() || p.reactions.size() > 2) return false;
+  int n_nl = 0;
+  for (const Reaction &q : p.reactions) {
+    if (q.type != MF_REACTION_NONLINEAR) continue;
+    ++n_nl;
+    const double bound = 0.5 * q.B * std::max(std::fabs(q.lt * q.lt - 1), std::fabs(q.lc * q.lc - 1));
+    if (!(bound <= 0.25)) return false;
+  }
+  return n_nl <= 1;
+}
+
 // Whether variant v can run RVE p (threads, registers, LDS, reactions,
 // energy diagnostics) under the batch parameters.
 bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE &p,
@@ -252,6 +267,7 @@ bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE
   if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) return false;
   if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) return false;
   if ((v.flags & kLinearOnly) && !lin) return false;
+  if ((v.flags & kMix2) && !mix2_ok(p)) return false;
   if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
     return false;
   if (regs && b->no_reg_variants) return false;
@@ -287,7 +303,9 @@ int make_plan(mf_batch *b, int prec) {
       const Variant &v = vs[i];
       if (!variant_fits(b, prec, v, p, lin)) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
-      long key = regs ? (long)v.npt * 10000000L + v.block
+      // reaction modes: linear-only, then one-linear-one-nonlinear, then table
+      const int mode_rank = (v.flags & kLinearOnly) ? 0 : (v.flags & kMix2) ? 1 : 2;
+      long key = regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
                       : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
         best = i;

@@ -149,21 +149,20 @@ __global__ __launch_bounds__(B) void orientation_kernel(
 static const Variant kVariants[] = {
     MF_VARIANT_LIST(MF_VARIANT_ENTRY)
     MF_VARIANT_LIST(MF_ENERGY_ENTRY)
-    // FP32c with incidence lists in registers (one free node per thread)
-    {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
-    {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
-    {MF_PREC_FP32C, 640, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
-    {MF_PREC_FP32C, 768, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
-    {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf | kLinearOnly},
+    // FP32c with incidence lists in registers, one free node per thread:
+    // linear-only and one-linear-one-nonlinear modes, reaction-table mode at
+    // 256 and 512 threads
+#define MF_REG2(B)                                                 \
+    {MF_PREC_FP32C, B, 1, kEllInRegs | kDoubleBuf | kLinearOnly},  \
+    {MF_PREC_FP32C, B, 1, kEllInRegs | kDoubleBuf | kMix2},
+    MF_REG2(256) MF_REG2(512) MF_REG2(640) MF_REG2(768) MF_REG2(1024)
+#undef MF_REG2
     {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf},
-    {MF_PREC_FP32C, 640, 1, kEllInRegs | kDoubleBuf},
-    {MF_PREC_FP32C, 768, 1, kEllInRegs | kDoubleBuf},
-    {MF_PREC_FP32C, 1024, 1, kEllInRegs | kDoubleBuf},
-    // several free nodes per thread (the spill-free instantiations)
+    // several free nodes per thread
     {MF_PREC_FP32C, 768, 2, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
-    {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf},
+    {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kMix2},
     {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},
 };
 
@@ -187,7 +186,8 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
   b += sizeof(double) * NW * kern::kRedStride;
   b += sizeof(double) * 6 * Nbp;
   if (v.flags & kEnergy) b += sizeof(double) * 3 * Nbp;
-  b += (sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15);
+  b += v.prec == MF_PREC_FP64 ? (sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15)
+                              : sizeof(float) * 4 * 3 * kMaxReactions;  // packed table
   if (v.flags & kEllInLds)
     b += (size_t)ell_free_entries *
          (sizeof(int32_t) + (v.prec == MF_PREC_FP64 ? sizeof(double) : sizeof(float)));

@@ -148,6 +148,57 @@ __device__ inline float react32_t(const ReactDev &q, float e2, float il) {
   return q.alpha * il * expm1f(q.B_half * e2);
 }
 
+// Branch-free form of react32_t over a reaction table repacked in LDS as
+// three float4 per reaction:
+//   [0] EA/2, alpha = EA/B, B/2, 1 if nonlinear else 0
+//   [1] lt^2 - 1, lt, Kt, bt        (tension extension, lambda > lt)
+//   [2] lc^2 - 1, lc, Kc, bc        (compression extension, lambda < lc)
+// In the toe region x = B/2 (lambda^2 - 1) lies in [B/2 (lc^2-1), B/2 (lt^2-1)];
+// clamping e2 to that range keeps the polynomial's argument inside it, and
+// the selected branch ignores the clamped value outside the region.
+constexpr float kToePolyMax = 0.25f;  // |x| bound of the degree-7 expm1
+constexpr int kReactTabStride = 3;    // float4 per reaction
+
+__device__ inline void pack_react_table(const ReactDev &q, float4 *t) {
+  const bool nl = q.type == MF_REACTION_NONLINEAR;
+  t[0] = make_float4(q.EA_half, nl ? q.alpha : 0.0f, nl ? q.B_half : 0.0f, nl ? 1.0f : 0.0f);
+  t[1] = make_float4(nl ? q.lt2m1 : 0.0f, q.lt_f, q.Kt, q.bt);
+  t[2] = make_float4(nl ? q.lc2m1 : 0.0f, q.lc_f, q.Kc, q.bc);
+}
+
+// expm1(x) for |x| <= kToePolyMax: Taylor series to x^7 (truncation below
+// x^8/8! = 3.8e-10 relative at the bound).
+__device__ inline float expm1_poly(float x) {
+  float p = 1.0f / 5040.0f;
+  p = p * x + 1.0f / 720.0f;
+  p = p * x + 1.0f / 120.0f;
+  p = p * x + 1.0f / 24.0f;
+  p = p * x + 1.0f / 6.0f;
+  p = p * x + 0.5f;
+  p = p * x + 1.0f;
+  return p * x;
+}
+
+// Branch-free: every lane evaluates both laws and both toe forms and selects.
+// expm1 on |x| <= kToePolyMax is the polynomial; beyond it exp(x) - 1 loses
+// at most a few ulp (e^x - 1 > 0.28 there).
+__device__ inline float react32_tab(const float4 *T, unsigned rid, float e2, float il) {
+  const float4 a = T[kReactTabStride * rid];
+  const float4 b = T[kReactTabStride * rid + 1], c = T[kReactTabStride * rid + 2];
+  const float tl = a.x * il * e2;
+  const bool hi = e2 > b.x, lo = e2 < c.x;
+  const float x = a.z * fminf(fmaxf(e2, c.x), b.x);
+  const float em = fabsf(x) <= kToePolyMax ? expm1_poly(x) : __expf(x) - 1.0f;
+  const float tt = a.y * il * em;
+  const float lam2 = 1.0f + e2;
+  const float ilam = __frsqrt_rn(lam2);
+  const float lam = lam2 * ilam;
+  const float K = hi ? b.z : c.z, L0 = hi ? b.y : c.y, bb = hi ? b.w : c.w;
+  const float te = (K * (lam - L0) + bb) * il * ilam;
+  const float tn = (hi || lo) ? te : tt;
+  return a.w != 0.0f ? tn : tl;
+}
+
 // Fast path when the whole RVE uses one linear reaction.
 struct Lin64 {
   double E, A;

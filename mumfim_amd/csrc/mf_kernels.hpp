@@ -54,6 +54,7 @@ enum : int32_t {
   kEllInRegs = 4,   // incidence lists in registers, one free node per thread
   kLinearOnly = 8,  // every fiber of the RVE uses one linear reaction
   kEnergy = 16,     // FP64 with energy diagnostics
+  kMix2 = 32,       // at most one linear and one nonlinear reaction (FP32c registers)
 };
 
 

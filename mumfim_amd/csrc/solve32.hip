@@ -46,8 +46,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   off += sizeof(double) * NW * kRedStride;
   double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
   off += sizeof(double) * 6 * Nbp;
-  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
-  off += align16(sizeof(ReactDev) * kMaxReactions);
+  float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
+  off += sizeof(float4) * kReactTabStride * kMaxReactions;
   float *ils = reinterpret_cast<float *>(smem + off);
   off += kEll ? sizeof(float) * R.ell_free_entries : 0;
   int32_t *ws = reinterpret_cast<int32_t *>(smem + off);
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   }
   const int32_t *Ew = kEll ? ws : gw;
   const float *Eil = kEll ? ils : gil;
-  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
   const bool lin_only =
       R.n_react == 1 && a.reacts[R.react_off].type == MF_REACTION_LINEAR;
   const float lin_EA_half = a.reacts[R.react_off].EA_half;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
         const float e2 = sl * (il * il);
         const float t = lin_only ? lin_EA_half * il * e2
-                                 : react32_t(rxs[(uint32_t)wv >> 24], e2, il);
+                                 : react32_tab(rtab, (uint32_t)wv >> 24, e2, il);
         fx -= dx * t;
         fy -= dy * t;
         fz -= dz * t;
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
       }
     }
-    if (ramp) {
+    if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B)
         Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
       const float e2 = sl * (il * il);
       const float t = lin_only ? lin_EA_half * il * e2
-                               : react32_t(rxs[(uint32_t)wv >> 24], e2, il);
+                               : react32_tab(rtab, (uint32_t)wv >> 24, e2, il);
       f1 -= dx * t;
       f2 -= dy * t;
       f3 -= dz * t;
@@ -409,8 +409,16 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 // ============================================================================
 constexpr int kLaneDeg = kLaneDegree;
 
-template <int B, int NPT, bool LIN>
+// Reaction modes: RM_TABLE looks each fiber's law up in the packed table;
+// RM_LIN (one linear reaction) keeps EA/(2 l0^3) per fiber; RM_MIX2 (at most
+// one linear and one nonlinear reaction, toe range within the expm1
+// polynomial's bound) keeps EA/(2 l0^3) for linear fibers and -1/l0 for
+// nonlinear ones, with the nonlinear law's constants uniform.
+enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
+
+template <int B, int NPT, int RM>
 __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
+  constexpr bool LIN = RM == RM_LIN;
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
@@ -428,14 +436,20 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   off += sizeof(double) * NW * kRedStride;
   double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
   off += sizeof(double) * 6 * Nbp;
-  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
+  float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
 
   const int32_t *gw = a.ell_w + R.ell_off;
   const float *gil = a.ell_il + R.ell_off;
   const double *gl0 = a.ell_l0 + R.ell_off;
   const double *X = a.X + 3 * nb;
-  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
   const double EA_half = 0.5 * a.reacts[R.react_off].E * a.reacts[R.react_off].A;
+  // RM_MIX2: the nonlinear reaction's constants (uniform)
+  ReactDev nlq;
+  if (RM == RM_MIX2) {
+    const int k = (R.n_react > 1 && a.reacts[R.react_off + 1].type == MF_REACTION_NONLINEAR) ? 1 : 0;
+    nlq = a.reacts[R.react_off + k];
+  }
 
   double F[9];
 #pragma unroll
@@ -529,9 +543,17 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         const int idx = base + (j0 + j) * kWave;
         const int32_t wv = gw[idx];
         const int w = wv & 0xFFFFFF;
-        offs[sl][j] = w * (int)sizeof(float4) | (LIN ? 0 : (wv & 0xFF000000));
+        offs[sl][j] = w * (int)sizeof(float4) | (RM == RM_TABLE ? (wv & 0xFF000000) : 0);
         const double l0 = gl0[idx];
-        cf[sl][j] = LIN ? (float)(EA_half / (l0 * l0 * l0)) : gil[idx];
+        if (RM == RM_LIN) {
+          cf[sl][j] = (float)(EA_half / (l0 * l0 * l0));
+        } else if (RM == RM_MIX2) {
+          const ReactDev &q = a.reacts[R.react_off + ((uint32_t)wv >> 24)];
+          cf[sl][j] = q.type == MF_REACTION_NONLINEAR ? -gil[idx]
+                                                      : (float)(0.5 * q.E * q.A / (l0 * l0 * l0));
+        } else {
+          cf[sl][j] = gil[idx];
+        }
         d0x[sl][j] = (float)(X[3 * w] - X[3 * r]);
         d0y[sl][j] = (float)(X[3 * w + 1] - X[3 * r + 1]);
         d0z[sl][j] = (float)(X[3 * w + 2] - X[3 * r + 2]);
@@ -560,7 +582,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
 #pragma unroll
       for (int j = 0; j < K; ++j)
         q[j] = *reinterpret_cast<const float4 *>(
-            P + (LIN ? offs[sl][j] : (offs[sl][j] & 0xFFFFFF)));
+            P + (RM == RM_TABLE ? (offs[sl][j] & 0xFFFFFF) : offs[sl][j]));
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         // both ends as their FP32-rounded displacements (the low parts only
@@ -573,11 +595,26 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         const float sq = (d0x[sl][j] + dx) * gx + (d0y[sl][j] + dy) * gy +
                          (d0z[sl][j] + dz) * gz;
         float t;
-        if (LIN) {
+        if (RM == RM_LIN) {
           t = cf[sl][j] * sq;
+        } else if (RM == RM_MIX2) {
+          // every lane evaluates both laws and selects (see react32_tab)
+          const float c = cf[sl][j];
+          const float il = -c;
+          const float e2 = sq * (il * il);
+          const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
+          const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
+          const float tt = nlq.alpha * il * expm1_poly(x);
+          const float lam2 = 1.0f + e2;
+          const float ilam = __frsqrt_rn(lam2);
+          const float lam = lam2 * ilam;
+          const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
+          const float bb = hi ? nlq.bt : nlq.bc;
+          const float te = (K * (lam - L0) + bb) * il * ilam;
+          t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
         } else {
           const float il = cf[sl][j];
-          t = react32_t(rxs[(uint32_t)offs[sl][j] >> 24], sq * (il * il), il);
+          t = react32_tab(rtab, (uint32_t)offs[sl][j] >> 24, sq * (il * il), il);
         }
         fx -= dx * t;
         fy -= dy * t;
@@ -679,7 +716,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         if (writer[sl]) Pn[rr[sl]] = make_float4(pc[0], pc[1], pc[2], 0.0f);
       }
     }
-    if (ramp) {
+    if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B)
         Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
@@ -781,7 +818,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
         t = (float)(EA_half / (l0 * l0 * l0)) * sq;
       } else {
         const float il = gil[idx];
-        t = react32_t(rxs[(uint32_t)wv >> 24], sq * (il * il), il);
+        t = react32_tab(rtab, (uint32_t)wv >> 24, sq * (il * il), il);
       }
       f1 -= dx * t;
       f2 -= dy * t;
@@ -841,28 +878,10 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
-#define MF_REG_LAUNCH(B, N)                                                    \
-  if (v.npt == N && v.block == B && (v.flags & kEllInRegs)) {                  \
-    auto k = (v.flags & kLinearOnly) ? kern::solve32r<B, N, true>              \
-                                     : kern::solve32r<B, N, false>;            \
-    if (lds > 64 * 1024)                                                       \
-      (void)hipFuncSetAttribute((const void *)k,                               \
-                                hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                (int)kMaxLds);                                 \
-    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                 \
-    return hipGetLastError();                                                  \
-  }
-  MF_REG_LAUNCH(256, 1)
-  MF_REG_LAUNCH(512, 1)
-  MF_REG_LAUNCH(640, 1)
-  MF_REG_LAUNCH(768, 1)
-  MF_REG_LAUNCH(1024, 1)
-#undef MF_REG_LAUNCH
-  // several nodes per thread: only the instantiations that stay spill-free
-#define MF_REG_LAUNCH(B, N, LIN)                                               \
+#define MF_REG_LAUNCH(B, N, RMODE, FLAG)                                       \
   if (v.npt == N && v.block == B && (v.flags & kEllInRegs) &&                  \
-      ((v.flags & kLinearOnly) != 0) == LIN) {                                 \
-    auto k = kern::solve32r<B, N, LIN>;                                        \
+      (v.flags & (kLinearOnly | kMix2)) == FLAG) {                             \
+    auto k = kern::solve32r<B, N, RMODE>;                                      \
     if (lds > 64 * 1024)                                                       \
       (void)hipFuncSetAttribute((const void *)k,                               \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,   \
@@ -870,10 +889,24 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
     hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                 \
     return hipGetLastError();                                                  \
   }
-  MF_REG_LAUNCH(768, 2, true)
-  MF_REG_LAUNCH(512, 3, true)
-  MF_REG_LAUNCH(512, 3, false)
-  MF_REG_LAUNCH(512, 4, true)
+#define MF_REG_LAUNCH2(B, N)                                                   \
+  MF_REG_LAUNCH(B, N, kern::RM_LIN, kLinearOnly)                               \
+  MF_REG_LAUNCH(B, N, kern::RM_MIX2, kMix2)
+  // only the instantiations that stay spill-free (the reaction-table mode
+  // holds the table rows of every fiber: 256 and 512 threads only)
+  MF_REG_LAUNCH(256, 1, kern::RM_TABLE, 0)
+  MF_REG_LAUNCH(512, 1, kern::RM_TABLE, 0)
+  MF_REG_LAUNCH2(256, 1)
+  MF_REG_LAUNCH2(512, 1)
+  MF_REG_LAUNCH2(640, 1)
+  MF_REG_LAUNCH2(768, 1)
+  MF_REG_LAUNCH2(1024, 1)
+  // several nodes per thread
+  MF_REG_LAUNCH(768, 2, kern::RM_LIN, kLinearOnly)
+  MF_REG_LAUNCH(512, 3, kern::RM_LIN, kLinearOnly)
+  MF_REG_LAUNCH(512, 3, kern::RM_MIX2, kMix2)
+  MF_REG_LAUNCH(512, 4, kern::RM_LIN, kLinearOnly)
+#undef MF_REG_LAUNCH2
 #undef MF_REG_LAUNCH
   return hipErrorInvalidValue;
 }

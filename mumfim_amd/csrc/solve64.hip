@@ -305,7 +305,7 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
         }
       }
     }
-    if (ramp) {
+    if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
 #pragma unroll
