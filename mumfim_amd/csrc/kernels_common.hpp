@@ -199,6 +199,42 @@ __device__ inline float react32_tab(const float4 *T, unsigned rid, float e2, flo
   return a.w != 0.0f ? tn : tl;
 }
 
+// One linear (EA/2 = ea_half_lin) and one nonlinear reaction (nl) with a
+// toe range within the polynomial's bound: t for a fiber of either law,
+// branch-free (see react32_tab).
+__device__ inline float react32_mix2(const ReactDev &nl, float ea_half_lin, bool is_nl,
+                                     float e2, float il) {
+  const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
+  const float x = nl.B_half * fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1);
+  const float tt = nl.alpha * il * expm1_poly(x);
+  const float lam2 = 1.0f + e2;
+  const float ilam = __frsqrt_rn(lam2);
+  const float lam = lam2 * ilam;
+  const float K = hi ? nl.Kt : nl.Kc, L0 = hi ? nl.lt_f : nl.lc_f, bb = hi ? nl.bt : nl.bc;
+  const float te = (K * (lam - L0) + bb) * il * ilam;
+  return is_nl ? ((hi || lo) ? te : tt) : ea_half_lin * il * e2;
+}
+
+// The reaction table qualifies for react32_mix2 (uniform over the
+// workgroup); nl_rid and ea_half_lin describe it.
+__device__ inline bool mix2_table(const ReactDev *q, int n, int &nl_rid, float &ea_half_lin) {
+  if (n < 1 || n > 2) return false;
+  int n_nl = 0;
+  nl_rid = -1;
+  ea_half_lin = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    if (q[i].type == MF_REACTION_NONLINEAR) {
+      ++n_nl;
+      nl_rid = i;
+      if (!(q[i].B_half * fmaxf(fabsf(q[i].lt2m1), fabsf(q[i].lc2m1)) <= kToePolyMax))
+        return false;
+    } else {
+      ea_half_lin = q[i].EA_half;
+    }
+  }
+  return n_nl == 1;
+}
+
 // Fast path when the whole RVE uses one linear reaction.
 struct Lin64 {
   double E, A;

@@ -63,6 +63,10 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   const int32_t *Ew = kEll ? ws : gw;
   const float *Eil = kEll ? ils : gil;
   if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
+  int nl_rid = -1;
+  float ea_half_lin = 0.0f;
+  const bool mix2 = mix2_table(a.reacts + R.react_off, R.n_react, nl_rid, ea_half_lin);
+  const ReactDev nlq = a.reacts[R.react_off + (nl_rid < 0 ? 0 : nl_rid)];
   const bool lin_only =
       R.n_react == 1 && a.reacts[R.react_off].type == MF_REACTION_LINEAR;
   const float lin_EA_half = a.reacts[R.react_off].EA_half;
@@ -72,9 +76,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
 
-  int Xv[NPT][3];
-  float uh[NPT][3], ul[NPT][3], vr[NPT][3], ar[NPT][3], imr[NPT];
-  int degr[NPT], Dr[NPT], baser[NPT];
+  float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
+  // slice width and base are wave-uniform (scalar registers)
+  int degr[NPT], Dr[NPT], sbase[NPT];
   // initial displacement: relax adds the affine increment (F - I) x to u,
   // ramp keeps u and sets the boundary increment aside.
   auto init_node = [&](int r, double (&u)[3]) {
@@ -118,21 +122,18 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     for (int d = 0; d < 3; ++d) {
       uh[s][d] = (float)u[d];
       ul[s][d] = (float)(u[d] - (double)uh[s][d]);
-      Xv[s][d] = r < N ? (int)rint(a.X[3 * (nb + r) + d] * R.xq_scale) : 0;
       vr[s][d] = 0.0f;
-      ar[s][d] = 0.0f;
     }
     degr[s] = 0;
-    Dr[s] = 0;
-    baser[s] = 0;
     imr[s] = 0.0f;
     if (r < Nf) {
       imr[s] = (float)(1.0 / a.mass[nb + r]);
       degr[s] = a.deg[nb + r];
-      const int k = r / kWave;
-      Dr[s] = a.slice_D[R.slice_off + k];
-      baser[s] = a.slice_base[R.slice_off + k] + lane;
     }
+    // lane 0 of the wave names the slice; waves past Nf never gather
+    const int k = r / kWave;
+    Dr[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
+    sbase[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
   }
   for (int r = NPT * B + tid; r < N; r += B) {
     double u[3];
@@ -145,20 +146,21 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   int cur = 0;
   auto node_force = [&](int s, int buf, float (&f)[3]) {
     const float4 *P = pos + (kDbuf ? buf * Np : 0);
+    const int4 Xv = Xs[s * B + tid];  // own reference coordinate
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    const int D = __builtin_amdgcn_readfirstlane(Dr[s]);
+    const int D = Dr[s];
     for (int j = 0; j < D; ++j) {
       if (j < degr[s]) {
-        const int idx = baser[s] + j * kWave;
+        const int idx = sbase[s] + lane + j * kWave;
         const int32_t wv = Ew[idx];
         const float il = Eil[idx];
         const int w = wv & 0xFFFFFF;
         const float4 q = P[w];
         const int4 xq = Xs[w];
         // reference vector: exact integer difference, one rounding to FP32
-        const float ex = (float)(xq.x - Xv[s][0]) * xq_inv;
-        const float ey = (float)(xq.y - Xv[s][1]) * xq_inv;
-        const float ez = (float)(xq.z - Xv[s][2]) * xq_inv;
+        const float ex = (float)(xq.x - Xv.x) * xq_inv;
+        const float ey = (float)(xq.y - Xv.y) * xq_inv;
+        const float ez = (float)(xq.z - Xv.z) * xq_inv;
         const float gx = (q.x - uh[s][0]) - ul[s][0];
         const float gy = (q.y - uh[s][1]) - ul[s][1];
         const float gz = (q.z - uh[s][2]) - ul[s][2];
@@ -166,8 +168,10 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         // l^2 - l0^2 = (d0 + d) . delta
         const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
         const float e2 = sl * (il * il);
+        const unsigned rid = (uint32_t)wv >> 24;
         const float t = lin_only ? lin_EA_half * il * e2
-                                 : react32_tab(rtab, (uint32_t)wv >> 24, e2, il);
+                        : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
+                                 : react32_tab(rtab, rid, e2, il);
         fx -= dx * t;
         fy -= dy * t;
         fz -= dz * t;
@@ -193,33 +197,43 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     return (double)lmin;
   };
 
-  // initialization: forces at the start positions, first time step
+  // The closing half kick of step k and the opening half kick of step k+1
+  // use the same acceleration: v_{k+1/2} = v_{k-1/2} + (h_k + h_{k+1}) a_k,
+  // applied as soon as a_k and h_{k+1} are known (v_{-1/2} = 0, h_{-1} = 0),
+  // so no acceleration is carried across the barrier.
+  const float crit_half = (float)(a.crit * 0.5);
+  float h_prev = 0.0f;
   float fint[NPT][3];
-  double resid_local = 0.0;
+  {
+    double resid_local = 0.0;
 #pragma unroll
-  for (int s = 0; s < NPT; ++s) {
-    if (s * B + tid < Nf) {
-      node_force(s, cur, fint[s]);
+    for (int s = 0; s < NPT; ++s)
+      if (s * B + tid < Nf) node_force(s, cur, fint[s]);
+    double lmin2 = min_length2(cur);
+    block_min_sum<B>(lmin2, resid_local, red);
+    const float h = crit_half * (sqrtf((float)lmin2) * inv_c);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) ar[s][d] = -fint[s][d] * imr[s] - eta * vr[s][d];
-    }
+    for (int s = 0; s < NPT; ++s)
+      if (s * B + tid < Nf) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + h * (-fint[s][d] * imr[s] - eta * vr[s][d]);
+      }
+    h_prev = h;
   }
-  double lmin2 = min_length2(cur);
-  block_min_sum<B>(lmin2, resid_local, red);
-  float dt_crit = sqrtf((float)lmin2) * inv_c;
 
   int step = 0;
   double residual = 10.0;
   int status = MF_STATUS_OK;
   const int freq = a.freq;
-  const float crit_half = (float)(a.crit * 0.5);
   for (;;) {
-    const float h = crit_half * dt_crit;
-    const float h2 = 2.0f * h;
+    const float h2 = 2.0f * h_prev;
     const int nxt = kDbuf ? cur ^ 1 : 0;
     float4 *Pn = pos + nxt * Np;
     if (!kDbuf) __syncthreads();  // all force reads of the last step done
-    // half kick + drift; u (hi, lo) += 2h v with TwoSum
+    // drift: u (hi, lo) += 2h v with Fast2Sum, exact when |u_hi| >= |increment|;
+    // otherwise u is within a few increments of zero and the uncompensated
+    // rounding is below eps * |increment|.
 #pragma unroll
     for (int s = 0; s < NPT; ++s) {
       const int r = s * B + tid;
@@ -227,13 +241,10 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         float pc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const float vl = vr[s][d] + h * ar[s][d];
-          vr[s][d] = vl;
-          const float y = ul[s][d] + h2 * vl;
-          float sh, e;
-          two_sum(uh[s][d], y, sh, e);
+          const float y = ul[s][d] + h2 * vr[s][d];
+          const float sh = uh[s][d] + y;
+          ul[s][d] = y - (sh - uh[s][d]);
           uh[s][d] = sh;
-          ul[s][d] = e;
           pc[d] = sh;
         }
         Pn[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
@@ -250,33 +261,34 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     __syncthreads();
     cur = nxt;
     const bool refresh = (step % freq) == 0;
+    // the step size follows the new positions and is known before the
+    // forces, so each node kicks as soon as its force is done
+    float h = h_prev;
+    if (refresh) {
+      double lm = min_length2(cur), none = 0.0;
+      block_min_sum<B>(lm, none, red);
+      h = crit_half * (sqrtf((float)lm) * inv_c);
+    }
+    const float hk = h_prev + h;
     double rsum = 0.0;
 #pragma unroll
     for (int s = 0; s < NPT; ++s) {
       if (s * B + tid < Nf) {
-        node_force(s, cur, fint[s]);
-        float rn = 0.0f;
+        float f[3];
+        node_force(s, cur, f);
+        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          ar[s][d] = -fint[s][d] * imr[s] - eta * vr[s][d];
-          rn += fint[s][d] * fint[s][d];
-        }
-        rsum += (double)rn;
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
       }
     }
     if (refresh) {
-      double lm = min_length2(cur);
-      block_min_sum<B>(lm, rsum, red);
-      dt_crit = sqrtf((float)lm) * inv_c;
+      // slots 2-3 of the scratch: waves may still read slots 0-1 above
+      double none = 0.0;
+      block_min_sum<B>(none, rsum, red + 2);
       residual = sqrt(rsum);
     }
-#pragma unroll
-    for (int s = 0; s < NPT; ++s) {
-      if (s * B + tid < Nf) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) vr[s][d] += h * ar[s][d];
-      }
-    }
+    h_prev = h;
     ++step;
     if (ramp) {
       if (step >= a.n_steps) break;
