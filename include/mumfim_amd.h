@@ -129,9 +129,18 @@ typedef struct mf_solver_params {
    * (every dt_update_freq steps after step 10, and at the end) stops with
    * status MF_STATUS_ENERGY. */
   int32_t energy_diagnostics;   /* 1: track energies                           */
-  int32_t pad_;
+  int32_t fp32_kernel;          /* MF_FP32_KERNEL_*: FP32c kernel family      */
   double energy_check_eps;      /* 0: no check                                 */
 } mf_solver_params;
+
+/* FP32c kernel families.  Node-centric kernels gather each free node's
+ * fibers (every fiber is evaluated from both ends); fiber-parallel kernels
+ * evaluate each fiber once and let nodes sum the published end forces (fewer
+ * arithmetic operations, more LDS traffic; RVEs up to ~1800 nodes).  AUTO
+ * lets the planner and autotuner choose per RVE. */
+#define MF_FP32_KERNEL_AUTO 0
+#define MF_FP32_KERNEL_NODE 1
+#define MF_FP32_KERNEL_FIBER 2
 
 /* Ramp-mode boundary amplitude alpha(xi), xi = k / n_steps. */
 #define MF_AMP_LINEAR 0         /* alpha = xi                                */
@@ -221,9 +230,12 @@ int mf_batch_set_state(mf_batch *b, const double *x, const double *u,
 /* Per-RVE sizes, 4 ints per RVE: nodes, free nodes, boundary nodes, fibers. */
 int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
 
-/* Kernel plan of the batch: for each RVE the variant chosen by the planner
- * (threads per solve, node slots per thread, incidence lists in LDS or not),
- * 4 ints per RVE.  Returns the number of distinct launch groups. */
+/* Kernel plan of the batch: for each RVE the variant chosen by the planner,
+ * 4 ints per RVE: threads per solve, free-node slots per thread, flags, LDS
+ * bytes.  Flags: 1 incidence lists in LDS, 2 double-buffered positions,
+ * 4 incidence lists in registers, 8 linear-only, 16 energy diagnostics,
+ * 32 one linear and one nonlinear reaction, 64 fiber-parallel (bits 8-11:
+ * fibers per thread).  Returns the number of distinct launch groups. */
 int mf_batch_plan(mf_batch *b, int32_t *plan);
 
 /* ---- device memory (no reference counterpart) ----------------------------- */

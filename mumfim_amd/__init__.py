@@ -26,6 +26,7 @@ MF_REACTION_LINEAR, MF_REACTION_NONLINEAR = 0, 1
 MF_STATUS_OK, MF_STATUS_NAN, MF_STATUS_STEP_CAP, MF_STATUS_FALLBACK = 0, 1, 2, 3
 MF_STATUS_ENERGY = 4
 MF_AMP_LINEAR, MF_AMP_SMOOTHSTEP = 0, 1
+MF_FP32_KERNEL_AUTO, MF_FP32_KERNEL_NODE, MF_FP32_KERNEL_FIBER = 0, 1, 2
 
 
 class mf_reaction(C.Structure):
@@ -52,7 +53,7 @@ class mf_solver_params(C.Structure):
                 ("visc_damp_coeff", C.c_double),
                 ("crit_time_scale_factor", C.c_double),
                 ("tolerance", C.c_double), ("fd_step", C.c_double),
-                ("energy_diagnostics", C.c_int32), ("pad_", C.c_int32),
+                ("energy_diagnostics", C.c_int32), ("fp32_kernel", C.c_int32),
                 ("energy_check_eps", C.c_double)]
 
 

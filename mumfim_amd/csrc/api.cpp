@@ -109,6 +109,8 @@ bool params_valid(const mf_solver_params &p) {
          (p.energy_diagnostics == 0 ||
           (p.energy_diagnostics == 1 && p.precision == MF_PREC_FP64)) &&
          p.energy_check_eps >= 0 &&
+         (p.fp32_kernel == MF_FP32_KERNEL_AUTO || p.fp32_kernel == MF_FP32_KERNEL_NODE ||
+          p.fp32_kernel == MF_FP32_KERNEL_FIBER) &&
          (p.block_size == 0 || p.block_size == 256 || p.block_size == 512 ||
           p.block_size == 640 || p.block_size == 768 || p.block_size == 1024);
 }
@@ -157,7 +159,7 @@ struct mf_batch {
   int last_launches = 0;
   DevBuf<RveDev> rves;
   DevBuf<double> X, mass, l0, V, cur_stress, F_in;
-  DevBuf<int32_t> deg, slice_base, slice_D, ell_w, fib;
+  DevBuf<int32_t> deg, slice_base, slice_D, ell_w, fib, fstream;
   DevBuf<double> ell_l0;
   DevBuf<float> ell_il;
   DevBuf<ReactDev> reacts;
@@ -271,6 +273,11 @@ bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE
   if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
     return false;
   if (regs && b->no_reg_variants) return false;
+  const bool fpar = (v.flags & kFiberPar) != 0;
+  if (fpar && (long)v.block * variant_fpl(v.flags) < p.M) return false;
+  if (prec == MF_PREC_FP32C && b->p.fp32_kernel != MF_FP32_KERNEL_AUTO &&
+      fpar != (b->p.fp32_kernel == MF_FP32_KERNEL_FIBER))
+    return false;
   return lds_bytes(v, p.N, p.Nb, p.ell_free_entries) <= kMaxLds;
 }
 
@@ -295,7 +302,8 @@ int make_plan(mf_batch *b, int prec) {
     const bool lin = all_linear(p);
     int best = -1;
     auto t = b->tuned.find(tune_key(prec, p, lin));
-    if (t != b->tuned.end() && !b->p.block_size && variant_fits(b, prec, vs[t->second], p, lin))
+    if (t != b->tuned.end() && !b->p.block_size && !b->p.fp32_kernel &&
+        variant_fits(b, prec, vs[t->second], p, lin))
       best = t->second;
     const bool use_tuned = best >= 0;
     long best_key = 0;
@@ -303,10 +311,17 @@ int make_plan(mf_batch *b, int prec) {
       const Variant &v = vs[i];
       if (!variant_fits(b, prec, v, p, lin)) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
+      const bool fpar = (v.flags & kFiberPar) != 0;
       // reaction modes: linear-only, then one-linear-one-nonlinear, then table
       const int mode_rank = (v.flags & kLinearOnly) ? 0 : (v.flags & kMix2) ? 1 : 2;
-      long key = regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
-                      : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
+      // register-resident lists with one node per thread; then, for RVEs
+      // with nonlinear fibers, fiber-parallel (fewest threads, fiber slots)
+      // before register lists with several nodes per thread, and after them
+      // for all-linear RVEs (measured order); then the LDS/global-list kernels
+      long key = fpar   ? (lin ? 50000000L : 15000000L) + (long)v.block * 1000 +
+                              (long)variant_fpl(v.flags) * 10 + mode_rank
+                 : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
+                        : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
         best = i;
         best_key = key;
@@ -327,7 +342,7 @@ int upload_batch(mf_batch *b) {
   const int R = b->R;
   std::vector<RveDev> rd(R);
   std::vector<double> X, mass, l0s, ell_l0;
-  std::vector<int32_t> deg, sb, sD, ew, fib;
+  std::vector<int32_t> deg, sb, sD, ew, fib, fst;
   std::vector<float> il;
   std::vector<ReactDev> rx;
   int64_t node_off = 0, ell_off = 0, fib_off = 0;
@@ -369,6 +384,7 @@ int upload_batch(mf_batch *b) {
     ell_l0.insert(ell_l0.end(), p.ell_l0.begin(), p.ell_l0.end());
     for (double v : p.ell_l0) il.push_back((float)(1.0 / v));
     fib.insert(fib.end(), p.fib.begin(), p.fib.end());
+    fst.insert(fst.end(), p.fstream.begin(), p.fstream.end());
     for (const Reaction &q : p.reactions) {
       ReactDev rdv;
       make_react_dev(q, rdv);
@@ -392,6 +408,7 @@ int upload_batch(mf_batch *b) {
   MF_HIP(b->ell_l0.upload(ell_l0, s));
   MF_HIP(b->ell_il.upload(il, s));
   MF_HIP(b->fib.upload(fib, s));
+  MF_HIP(b->fstream.upload(fst, s));
   MF_HIP(b->reacts.upload(rx, s));
   // state: x = X, u = 0, V = 1 (constructor, ...Explicit.h:115-118,250-253)
   for (int k = 0; k < 2; ++k) {
@@ -517,6 +534,7 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.ell_l0 = b->ell_l0.p;
   a.ell_il = b->ell_il.p;
   a.fib = b->fib.p;
+  a.fstream = b->fstream.p;
   a.reacts = b->reacts.p;
   a.x_in = x_in;
   a.u_in = u_in;
@@ -568,6 +586,7 @@ void fill_args(mf_batch *b, SolveArgs &a) {
   a.ell_l0 = b->ell_l0.p;
   a.ell_il = b->ell_il.p;
   a.fib = b->fib.p;
+  a.fstream = b->fstream.p;
   a.reacts = b->reacts.p;
   a.V = b->V.p;
   a.steps = b->steps.p;
@@ -1044,11 +1063,13 @@ int mf_batch_set_params(mf_batch *b, const mf_solver_params *params) {
   const bool mode_changed = params->mode != b->p.mode;
   const int old_bs = b->p.block_size;
   const int old_en = b->p.energy_diagnostics;
+  const int old_fk = b->p.fp32_kernel;
   b->p = *params;
   if (mode_changed) {
     b->have_prev = false;
   }
-  if (old_bs != params->block_size || old_en != params->energy_diagnostics)
+  if (old_bs != params->block_size || old_en != params->energy_diagnostics ||
+      old_fk != params->fp32_kernel)
     for (int prec = 0; prec < 2; ++prec) {
       int e = make_plan(b, prec);
       if (e != MF_OK) return e;
@@ -1070,8 +1091,8 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
     set_error("mf_batch_autotune: bad arguments");
     return MF_ERR_INVALID;
   }
-  if (b->p.block_size) {
-    set_error("mf_batch_autotune: block_size forces the variant; set it to 0");
+  if (b->p.block_size || b->p.fp32_kernel) {
+    set_error("mf_batch_autotune: block_size or fp32_kernel forces the variant; set them to 0");
     return MF_ERR_INVALID;
   }
   const int prec = precision;

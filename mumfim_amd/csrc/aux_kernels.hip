@@ -1,6 +1,8 @@
 // Small FP64 kernels around the solve (volume update, finite-difference
 // tangent columns, orientation tensors) and the kernel-variant table.
 // Compiled with -ffp-contract=off.
+#include <algorithm>
+
 #include "kernels_common.hpp"
 
 namespace mf {
@@ -164,6 +166,10 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kMix2},
     {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},
+    // fiber-parallel: (threads, fibers per thread, free nodes per thread)
+#define MF_FPAR_ENTRY(B, FPL, NPL, FL) {MF_PREC_FP32C, B, NPL, kFiberPar | FL | ((FPL) << kFplShift)},
+    MF_FIBER_PAR_LIST(MF_FPAR_ENTRY)
+#undef MF_FPAR_ENTRY
 };
 
 const Variant *variants(int *count) {
@@ -177,6 +183,10 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
   size_t b = 0;
   if (v.prec == MF_PREC_FP64) {
     b += sizeof(double) * 3 * Np * ((v.flags & kDoubleBuf) ? 2 : 1);
+  } else if (v.flags & kFiberPar) {
+    // displacements + end-force slots (a spare per lane), which hold the
+    // low parts at the end
+    b += 16 * (Np + std::max<size_t>((size_t)ell_free_entries + kern::kWave, Np));
   } else if (v.flags & kEllInRegs) {
     b += 16 * Np * 2;  // displacement buffers only
   } else {

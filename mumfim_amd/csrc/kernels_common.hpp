@@ -243,6 +243,16 @@ struct Lin64 {
 }  // namespace kern
 }  // namespace mf
 
+// Fiber-parallel FP32c variants: (threads, fibers per thread, free nodes per
+// thread, reaction mode flags kLinearOnly|kMix2, 0 = reaction table).
+#define MF_FIBER_PAR_LIST(X)                                                 \
+  X(512, 2, 1, kLinearOnly) X(512, 2, 1, kMix2) X(512, 2, 1, 0)              \
+  X(512, 3, 1, kLinearOnly) X(512, 3, 1, kMix2) X(512, 3, 1, 0)              \
+  X(768, 2, 1, kLinearOnly) X(768, 2, 1, kMix2)                              \
+  X(1024, 2, 1, kLinearOnly) X(1024, 2, 1, kMix2)                            \
+  X(1024, 3, 2, kLinearOnly) X(1024, 3, 2, kMix2)                            \
+  X(1024, 4, 2, kLinearOnly) X(1024, 4, 2, kMix2)
+
 // Compiled kernel variants: (precision, threads, free-node slots per thread,
 // flags kEllInLds|kDoubleBuf).
 #define MF_VARIANT_LIST(X)                                                   \

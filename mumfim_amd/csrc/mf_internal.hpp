@@ -84,6 +84,12 @@ struct PackedRVE {
   int ell_free_entries = 0;            // entries of slices that hold free nodes
   int max_free_deg = 0;                // largest degree of a free node
   int n_split = 0;                     // free nodes with degree > kLaneDegree
+  // Fiber stream of the fiber-parallel kernel, 4 per fiber: ranks (a, b)
+  // and the ELL entries of the fiber in a's and b's lists (-1 at a boundary
+  // end).  Fibers are taken in ELL entry order at their first free end, so
+  // consecutive fibers mostly have lane-consecutive a; fibers with two
+  // boundary ends come last.
+  std::vector<int32_t> fstream;
   std::vector<int32_t> fib;            // 2M kernel ranks, input fiber order
   std::vector<double> l0;              // M
   std::vector<int32_t> rid;            // M (after the reaction policy)

@@ -23,6 +23,7 @@ struct SolveArgs {
   const double *ell_l0;     // FP64 kernels
   const float *ell_il;      // FP32 kernels: 1/l0
   const int32_t *fib;       // 2 per fiber (kernel ranks)
+  const int32_t *fstream;   // 4 per fiber: ranks a, b and their ELL entries
   const ReactDev *reacts;
   const double *x_in, *u_in;
   double *x_out, *u_out;
@@ -55,7 +56,11 @@ enum : int32_t {
   kLinearOnly = 8,  // every fiber of the RVE uses one linear reaction
   kEnergy = 16,     // FP64 with energy diagnostics
   kMix2 = 32,       // at most one linear and one nonlinear reaction (FP32c registers)
+  kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
 };
+// Fibers per thread of a fiber-parallel variant (flags bits 8-11).
+constexpr int kFplShift = 8;
+inline int variant_fpl(int32_t flags) { return (flags >> kFplShift) & 15; }
 
 
 struct Variant {

@@ -379,13 +379,15 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
   std::vector<int> row(N + 1, 0);
   for (int r = 0; r < N; ++r) row[r + 1] = row[r] + p.deg[r];
   std::vector<int> fillp(row.begin(), row.end() - 1);
-  std::vector<int32_t> csr_w(row[N]);
+  std::vector<int32_t> csr_w(row[N]), csr_e(row[N]);
   std::vector<double> csr_l0(row[N]);
   for (int e = 0; e < M; ++e) {
     int a = p.fib[2 * e], c = p.fib[2 * e + 1];
     int32_t tag = p.rid[e] << 24;
+    csr_e[fillp[a]] = e;
     csr_w[fillp[a]] = c | tag;
     csr_l0[fillp[a]++] = p.l0[e];
+    csr_e[fillp[c]] = e;
     csr_w[fillp[c]] = a | tag;
     csr_l0[fillp[c]++] = p.l0[e];
   }
@@ -415,6 +417,35 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
       p.ell_l0[at] = csr_l0[row[r] + j];
     }
   }
+  // fiber stream: ELL entry of every free fiber end, then fibers in entry
+  // order at their first free end, then boundary-only fibers
+  std::vector<int32_t> end_slot(2 * (size_t)M, -1);
+  for (int r = 0; r < p.Nf; ++r) {
+    int k = r / kSlice, lane = r % kSlice;
+    for (int j = 0; j < p.deg[r]; ++j) {
+      const int e = csr_e[row[r] + j];
+      end_slot[2 * (size_t)e + (p.fib[2 * e] == r ? 0 : 1)] =
+          p.slice_base[k] + j * kSlice + lane;
+    }
+  }
+  p.fstream.clear();
+  p.fstream.reserve(4 * (size_t)M);
+  std::vector<char> taken(M, 0);
+  for (int k = 0; k < n_free_slices; ++k)
+    for (int j = 0; j < p.slice_D[k]; ++j)
+      for (int lane = 0; lane < kSlice; ++lane) {
+        const int r = k * kSlice + lane;
+        if (r >= p.Nf || j >= p.deg[r]) continue;
+        const int e = csr_e[row[r] + j];
+        if (taken[e]) continue;
+        taken[e] = 1;
+        const int end = p.fib[2 * e] == r ? 0 : 1;
+        const int o = p.fib[2 * e + 1 - end];
+        p.fstream.insert(p.fstream.end(),
+                         {r, o, end_slot[2 * (size_t)e + end], end_slot[2 * (size_t)e + 1 - end]});
+      }
+  for (int e = 0; e < M; ++e)
+    if (!taken[e]) p.fstream.insert(p.fstream.end(), {p.fib[2 * e], p.fib[2 * e + 1], -1, -1});
   return MF_OK;
 }
 

@@ -873,6 +873,474 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   }
 }
 
+// ============================================================================
+// FP32c fiber-parallel kernel
+// ============================================================================
+// Lanes own fibers: each fiber is evaluated once per step and publishes its
+// two end forces, -t d to end a and +t d to end b, into LDS slots laid out as
+// the sliced ELL (the entry of the fiber in each end's incidence list; ends
+// at boundary nodes write a spare slot of their lane, so they never contend
+// for one address).  Free nodes then sum their own
+// slots, which sit lane-consecutive, kick and drift.  Two barriers per step:
+// positions published, end forces published.  The per-fiber arithmetic is
+// the register kernel's (both ends as FP32-rounded displacements, exactly
+// rounded reference vectors), done once per fiber instead of once per end.
+// Slots that no fiber owns (ELL padding) stay zero, so nodes sum all entries
+// of their slice width unmasked.
+template <int B, int FPL, int NPL, int RM>
+__global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
+  constexpr bool LIN = RM == RM_LIN;
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int ne = R.ell_free_entries;
+  const int nslot = max(ne + kWave, Np);
+  const int64_t nb = R.node_off;
+
+  size_t off = 0;
+  float4 *pos = reinterpret_cast<float4 *>(smem);  // [Np] displacement hi
+  off += sizeof(float4) * Np;
+  float4 *fsl = reinterpret_cast<float4 *>(smem + off);  // [nslot] end forces
+  off += sizeof(float4) * nslot;
+  double *red = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  off += sizeof(double) * 6 * Nbp;
+  float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const float *gil = a.ell_il + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  const double *X = a.X + 3 * nb;
+  if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
+  const double EA_half = 0.5 * a.reacts[R.react_off].E * a.reacts[R.react_off].A;
+  ReactDev nlq;
+  if (RM == RM_MIX2) {
+    const int k = (R.n_react > 1 && a.reacts[R.react_off + 1].type == MF_REACTION_NONLINEAR) ? 1 : 0;
+    nlq = a.reacts[R.react_off + k];
+  }
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  // initial displacement: relax adds the affine increment (F - I) x to u,
+  // ramp keeps u and sets the boundary increment aside
+  auto init_u = [&](int r, double (&u)[3], double (&inc)[3]) {
+    const int64_t g = nb + r;
+    double xp[3], up[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+      u[k] = ramp ? up[k] : inc[k] + up[k];
+    }
+  };
+  for (int r = tid; r < N; r += B) {
+    double u[3], inc[3];
+    init_u(r, u, inc);
+    pos[r] = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = u[k];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  }
+  for (int i = tid; i < nslot; i += B) fsl[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+  // free nodes: r = s * B + tid
+  float uh[NPL][3], ul[NPL][3], vr[NPL][3], imr[NPL];
+  int degn[NPL], Dn[NPL], sbn[NPL];
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    imr[s] = 0.0f;
+    degn[s] = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uh[s][d] = ul[s][d] = vr[s][d] = 0.0f;
+    if (r < Nf) {
+      double u[3], inc[3];
+      init_u(r, u, inc);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        uh[s][d] = (float)u[d];
+        ul[s][d] = (float)(u[d] - (double)uh[s][d]);
+      }
+      imr[s] = (float)(1.0 / a.mass[nb + r]);
+      degn[s] = a.deg[nb + r];
+    }
+    // lane 0 of the wave names the slice; waves past Nf never gather
+    const int k = r / kWave;
+    Dn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
+    sbn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
+  }
+
+  // fibers: e = s * B + tid; byte offsets of the end positions (reaction id
+  // in the top byte in table mode) and of the end-force slots
+  int pa[FPL], pb[FPL], fa[FPL], fb[FPL];
+  float d0x[FPL], d0y[FPL], d0z[FPL], cf[FPL];
+  const int32_t *fs = a.fstream + 4 * R.fib_off;
+#pragma unroll
+  for (int s = 0; s < FPL; ++s) {
+    const int e = s * B + tid;
+    pa[s] = pb[s] = 0;
+    fa[s] = fb[s] = (ne + lane) * (int)sizeof(float4);
+    d0x[s] = d0y[s] = d0z[s] = 0.0f;
+    cf[s] = 0.0f;
+    if (e < M) {
+      const int4 q = *reinterpret_cast<const int4 *>(fs + 4 * e);
+      pa[s] = q.x * (int)sizeof(float4);
+      pb[s] = q.y * (int)sizeof(float4);
+      if (q.z >= 0) fa[s] = q.z * (int)sizeof(float4);
+      if (q.w >= 0) fb[s] = q.w * (int)sizeof(float4);
+      d0x[s] = (float)(X[3 * q.y] - X[3 * q.x]);
+      d0y[s] = (float)(X[3 * q.y + 1] - X[3 * q.x + 1]);
+      d0z[s] = (float)(X[3 * q.y + 2] - X[3 * q.x + 2]);
+      if (q.z >= 0) {  // fibers with two boundary ends only set the time step
+        const double l0 = gl0[q.z];
+        const int32_t wv = gw[q.z];
+        if (RM == RM_LIN) {
+          cf[s] = (float)(EA_half / (l0 * l0 * l0));
+        } else if (RM == RM_MIX2) {
+          const ReactDev &rq = a.reacts[R.react_off + ((uint32_t)wv >> 24)];
+          cf[s] = rq.type == MF_REACTION_NONLINEAR ? -gil[q.z]
+                                                   : (float)(0.5 * rq.E * rq.A / (l0 * l0 * l0));
+        } else {
+          cf[s] = gil[q.z];
+          pa[s] |= wv & 0xFF000000;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const float inv_c = (float)(1.0 / R.c_sound);
+  const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
+
+  // end forces of every owned fiber into the slots; squared current length
+  // minimum over the owned fibers when asked
+  // waves whose fibers of a slot all lie past M skip that slot
+  bool live[FPL];
+#pragma unroll
+  for (int s = 0; s < FPL; ++s)
+    live[s] = __builtin_amdgcn_readfirstlane(s * B + (tid & ~(kWave - 1))) < M;
+  auto fiber_phase = [&](bool want_min) {
+    const unsigned char *P = reinterpret_cast<const unsigned char *>(pos);
+    unsigned char *Fs = reinterpret_cast<unsigned char *>(fsl);
+    float lmin = 3.0e38f;
+    float4 qa[FPL], qb[FPL];
+#pragma unroll
+    for (int s = 0; s < FPL; ++s)
+      if (live[s]) {
+        qa[s] = *reinterpret_cast<const float4 *>(P + (RM == RM_TABLE ? (pa[s] & 0xFFFFFF) : pa[s]));
+        qb[s] = *reinterpret_cast<const float4 *>(P + pb[s]);
+      }
+#pragma unroll
+    for (int s = 0; s < FPL; ++s) {
+      if (!live[s]) continue;
+      const float gx = qb[s].x - qa[s].x;
+      const float gy = qb[s].y - qa[s].y;
+      const float gz = qb[s].z - qa[s].z;
+      const float dx = d0x[s] + gx, dy = d0y[s] + gy, dz = d0z[s] + gz;
+      // l^2 - l0^2 = (d0 + d) . delta
+      const float sq = (d0x[s] + dx) * gx + (d0y[s] + dy) * gy + (d0z[s] + dz) * gz;
+      float t;
+      if (RM == RM_LIN) {
+        t = cf[s] * sq;
+      } else if (RM == RM_MIX2) {
+        const float c = cf[s];
+        const float il = -c;
+        const float e2 = sq * (il * il);
+        const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
+        const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
+        const float tt = nlq.alpha * il * expm1_poly(x);
+        const float lam2 = 1.0f + e2;
+        const float ilam = __frsqrt_rn(lam2);
+        const float lam = lam2 * ilam;
+        const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
+        const float bb = hi ? nlq.bt : nlq.bc;
+        const float te = (K * (lam - L0) + bb) * il * ilam;
+        t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
+      } else {
+        const float il = cf[s];
+        t = react32_tab(rtab, (uint32_t)pa[s] >> 24, sq * (il * il), il);
+      }
+      if (want_min && s * B + tid < M) lmin = fminf(lmin, dx * dx + dy * dy + dz * dz);
+      const float fx = dx * t, fy = dy * t, fz = dz * t;
+      *reinterpret_cast<float4 *>(Fs + fa[s]) = make_float4(-fx, -fy, -fz, 0.0f);
+      *reinterpret_cast<float4 *>(Fs + fb[s]) = make_float4(fx, fy, fz, 0.0f);
+    }
+    return lmin;
+  };
+  // internal force of a free node: the sum of its slots.  The wave-uniform
+  // slice width selects a straight-line body so the loads issue together.
+  auto node_force = [&](int s, float (&f)[3]) {
+    const float4 *Q = fsl + sbn[s] + lane;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    // loads in batches of at most 6 rows (bounded registers)
+    auto batch = [&](auto J0C, auto KC) {
+      constexpr int J0 = decltype(J0C)::value, K = decltype(KC)::value;
+      float4 q[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) q[j] = Q[(J0 + j) * kWave];
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        fx += q[j].x;
+        fy += q[j].y;
+        fz += q[j].z;
+      }
+    };
+    auto body = [&](auto KC) {
+      constexpr int K = decltype(KC)::value;
+      batch(std::integral_constant<int, 0>{}, std::integral_constant<int, (K < 6 ? K : 6)>{});
+      if constexpr (K > 6)
+        batch(std::integral_constant<int, 6>{}, std::integral_constant<int, K - 6>{});
+    };
+    switch (Dn[s]) {
+#define MF_DEG_CASE(k) \
+  case k:              \
+    body(std::integral_constant<int, k>{}); \
+    break;
+      MF_DEG_CASE(1) MF_DEG_CASE(2) MF_DEG_CASE(3) MF_DEG_CASE(4)
+      MF_DEG_CASE(5) MF_DEG_CASE(6) MF_DEG_CASE(7) MF_DEG_CASE(8)
+      MF_DEG_CASE(9) MF_DEG_CASE(10) MF_DEG_CASE(11) MF_DEG_CASE(12)
+#undef MF_DEG_CASE
+      default:
+        for (int j = 0; j < Dn[s]; ++j) {
+          const float4 q = Q[j * kWave];
+          fx += q.x;
+          fy += q.y;
+          fz += q.z;
+        }
+        break;
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  auto wave_min_to = [&](float v, double *slot) {
+    double m = wave_min((double)v);
+    if (lane == 0) slot[(tid / kWave) * kRedStride] = m;
+  };
+  auto red_min = [&](const double *slot) {
+    double m = slot[0];
+#pragma unroll
+    for (int k = 1; k < NW; ++k) m = fmin(m, slot[k * kRedStride]);
+    return m;
+  };
+
+  // The closing half kick of step k and the opening half kick of step k+1
+  // use the same acceleration: v_{k+1/2} = v_{k-1/2} + (h_k + h_{k+1}) a_k,
+  // applied as soon as a_k and h_{k+1} are known (v_{-1/2} = 0, h_{-1} = 0).
+  const float crit_half = (float)(a.crit * 0.5);
+  float h_prev = 0.0f;
+  {
+    const float lm = fiber_phase(true);
+    wave_min_to(lm, red);
+    __syncthreads();
+    const float h = crit_half * (sqrtf((float)red_min(red)) * inv_c);
+#pragma unroll
+    for (int s = 0; s < NPL; ++s)
+      if (s * B + tid < Nf) {
+        float f[3];
+        node_force(s, f);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + h * (-f[d] * imr[s] - eta * vr[s][d]);
+      }
+    h_prev = h;
+  }
+
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  for (;;) {
+    const float h2 = 2.0f * h_prev;
+    // drift: u (hi, lo) += 2h v with Fast2Sum, exact when |u_hi| >= |increment|;
+    // otherwise u is within a few increments of zero and the uncompensated
+    // rounding is below eps * |increment|.
+#pragma unroll
+    for (int s = 0; s < NPL; ++s) {
+      const int r = s * B + tid;
+      if (r < Nf) {
+        float pc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float y = ul[s][d] + h2 * vr[s][d];
+          const float sh = uh[s][d] + y;
+          ul[s][d] = y - (sh - uh[s][d]);
+          uh[s][d] = sh;
+          pc[d] = sh;
+        }
+        pos[r] = make_float4(pc[0], pc[1], pc[2], 0.0f);
+      }
+    }
+    if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
+      for (int b = tid; b < Nb; b += B)
+        pos[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
+                                  (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
+                                  (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
+                                  0.0f);
+    }
+    __syncthreads();  // positions published; every slot read of the last step done
+    const bool refresh = (step % freq) == 0;
+    const float lm = fiber_phase(refresh);
+    if (refresh) wave_min_to(lm, red);
+    __syncthreads();  // end forces published
+    float h = h_prev;
+    if (refresh) h = crit_half * (sqrtf((float)red_min(red)) * inv_c);
+    const float hk = h_prev + h;
+    double rsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NPL; ++s)
+      if (s * B + tid < Nf) {
+        float f[3];
+        node_force(s, f);
+        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
+      }
+    if (refresh) {
+      // residual in slots 1 of the scratch (slots 0 are read above)
+      const double ws = wave_sum(rsum);
+      if (lane == 0) red[(tid / kWave) * kRedStride + 1] = ws;
+      __syncthreads();
+      double sum = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) sum += red[k * kRedStride + 1];
+      residual = sqrt(sum);
+    }
+    h_prev = h;
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // finalize: low parts into the slot buffer, boundary gathers in FP32 with
+  // exactly rounded reference vectors, stress in FP64
+  float4 *Ph = pos;
+  float4 *Pl = fsl;
+  __syncthreads();  // every slot and scratch read is done
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) Pl[r] = make_float4(ul[s][0], ul[s][1], ul[s][2], 0.0f);
+  }
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    double ub[3];
+    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+    Ph[Nf + b] = hb;
+    Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
+                             (float)(ub[2] - (double)hb.z), 0.0f);
+  }
+  __syncthreads();
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const int flags = a.solve_flags[sid];
+  for (int b = tid; b < Nb; b += B) {
+    const int rb = Nf + b;
+    const int64_t gr = nb + rb;
+    double ub[3], xb[3];
+    for (int d = 0; d < 3; ++d) {
+      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+      xb[d] = X[3 * rb + d] + ub[d];
+    }
+    const float4 hb = Ph[rb], lb = Pl[rb];
+    const int k = rb / kWave;
+    const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
+    const int dg = a.deg[gr];
+    float f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const int w = wv & 0xFFFFFF;
+      const float4 qh = Ph[w], ql = Pl[w];
+      const float ex = (float)(X[3 * w] - X[3 * rb]);
+      const float ey = (float)(X[3 * w + 1] - X[3 * rb + 1]);
+      const float ez = (float)(X[3 * w + 2] - X[3 * rb + 2]);
+      const float gx = (qh.x - hb.x) + (ql.x - lb.x);
+      const float gy = (qh.y - hb.y) + (ql.y - lb.y);
+      const float gz = (qh.z - hb.z) + (ql.z - lb.z);
+      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      float t;
+      if (LIN) {
+        const double l0 = gl0[idx];
+        t = (float)(EA_half / (l0 * l0 * l0)) * sq;
+      } else {
+        const float il = gil[idx];
+        t = react32_tab(rtab, (uint32_t)wv >> 24, sq * (il * il), il);
+      }
+      f1 -= dx * t;
+      f2 -= dy * t;
+      f3 -= dz * t;
+    }
+    const double g1 = f1, g2 = f2, g3 = f3;
+    st[0] += xb[0] * g1;
+    st[1] += xb[1] * g2;
+    st[2] += xb[2] * g3;
+    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
+    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
+    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
+  }
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) {
+      const int64_t g = nb + r;
+      for (int d = 0; d < 3; ++d) {
+        const double u = (double)uh[s][d] + (double)ul[s][d];
+        if (flags & kWriteX) a.x_out[3 * g + d] = X[3 * r + d] + u;
+        if (flags & kWriteU) a.u_out[3 * g + d] = u;
+      }
+    }
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !isfinite(v);
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
 }  // namespace kern
 
 hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
@@ -920,6 +1388,22 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   MF_REG_LAUNCH(512, 4, kern::RM_LIN, kLinearOnly)
 #undef MF_REG_LAUNCH2
 #undef MF_REG_LAUNCH
+#define MF_FPAR_LAUNCH(B, FPL, NPL, FL)                                       \
+  if (v.block == B && v.npt == NPL &&                                         \
+      v.flags == (kFiberPar | (FL) | ((FPL) << kFplShift))) {                 \
+    auto k = kern::solve32f<B, FPL, NPL,                                      \
+                            (FL) == kLinearOnly ? kern::RM_LIN                \
+                            : (FL) == kMix2     ? kern::RM_MIX2               \
+                                                : kern::RM_TABLE>;            \
+    if (lds > 64 * 1024)                                                      \
+      (void)hipFuncSetAttribute((const void *)k,                              \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                (int)kMaxLds);                                \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                \
+    return hipGetLastError();                                                 \
+  }
+  MF_FIBER_PAR_LIST(MF_FPAR_LAUNCH)
+#undef MF_FPAR_LAUNCH
   return hipErrorInvalidValue;
 }
 

@@ -4,6 +4,7 @@ same seeded networks and deformation gradients.
 Tolerances: FP64 path <= 1e-10 relative stress error (relative to the largest
 stress component of each RVE); FP32 + compensated path <= 1e-4 relative.
 """
+import dataclasses
 import math
 import os
 import sys
@@ -314,6 +315,46 @@ def test_register_kernel_slots_agree_bitwise():
         err = rel_err(ref, o.run(F))
         say(f"  vs oracle {err:.2e}")
         assert err <= FP32C_TOL
+
+
+def test_fiber_parallel_kernel_matches_oracle():
+    """The fiber-parallel FP32c kernel (each fiber evaluated once, nodes sum
+    the published end forces) in its linear, one-linear-one-nonlinear and
+    reaction-table modes: within the FP32c bar of the oracle, and close to
+    the node-centric kernels on the same solves (the two round the per-fiber
+    force contributions differently)."""
+    say("case: fiber-parallel kernel, three reaction modes")
+    nets = [mf.generate_network(seed=80 + i, n_nodes=n, nonlinear_fraction=nl)
+            for i, (n, nl) in enumerate([(480, 0.0), (1150, 0.5), (1400, 0.5)])]
+    # a second nonlinear reaction (B = 2) puts the RVE in reaction-table mode
+    t = mf.generate_network(seed=90, n_nodes=480, nonlinear_fraction=0.5)
+    t.reactions.append(dataclasses.replace(t.reactions[1], B=2.0))
+    t.fiber_reaction = (np.arange(t.n_fibers) % 3).astype(np.int32)
+    nets.append(t)
+    F = strain_F(8, len(nets), 0.08)
+    common = dict(n_steps=1500, reaction_policy=mf.MF_REACT_PER_FIBER)
+    fib = mf.BatchedFiberRVEAnalysisExplicit(
+        nets, ramp_params(precision=mf.MF_PREC_FP32C, fp32_kernel=mf.MF_FP32_KERNEL_FIBER, **common))
+    _, plan = fib.plan()
+    say(f"  fiber plan {plan[:, :3].tolist()}")
+    assert np.all(plan[:, 2] & 64), plan
+    modes = [int(f) & (8 | 32) for f in plan[:, 2]]
+    assert modes[0] == 8 and modes[1] == 32 and modes[2] == 32 and modes[3] == 0, modes
+    s_f, st = fib.run(F, return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
+    node = mf.BatchedFiberRVEAnalysisExplicit(
+        nets, ramp_params(precision=mf.MF_PREC_FP32C, fp32_kernel=mf.MF_FP32_KERNEL_NODE, **common))
+    _, nplan = node.plan()
+    assert not np.any(nplan[:, 2] & 64), nplan
+    s_n = node.run(F)
+    s_o = Oracle(nets, ramp_params(**common)).run(F)
+    e_f, e_n, e_fn = rel_err(s_f, s_o), rel_err(s_n, s_o), rel_err(s_f, s_n)
+    say(f"  fiber vs oracle {e_f:.2e}, node vs oracle {e_n:.2e}, fiber vs node {e_fn:.2e}")
+    assert e_f <= FP32C_TOL and e_n <= FP32C_TOL
+    # committed displacements of the two families agree to FP32 accumulation
+    _, u_f, _ = fib.get_state()
+    _, u_n, _ = node.get_state()
+    assert np.abs(u_f - u_n).max() <= 1e-4 * np.abs(u_n).max()
 
 
 def test_energy_diagnostics_and_smoothstep_match_oracle(net27):
