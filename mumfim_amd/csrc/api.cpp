@@ -140,6 +140,10 @@ struct TuneKey {
     return std::tie(prec, lin, regok, lanes64, n256) <
            std::tie(o.prec, o.lin, o.regok, o.lanes64, o.n256);
   }
+  bool operator==(const TuneKey &o) const {
+    return std::tie(prec, lin, regok, lanes64, n256) ==
+           std::tie(o.prec, o.lin, o.regok, o.lanes64, o.n256);
+  }
 };
 
 struct mf_batch {
@@ -159,9 +163,9 @@ struct mf_batch {
   int last_launches = 0;
   DevBuf<RveDev> rves;
   DevBuf<double> X, mass, l0, V, cur_stress, F_in;
-  DevBuf<int32_t> deg, slice_base, slice_D, ell_w, fib, fstream;
+  DevBuf<int32_t> deg, slice_base, slice_D, ell_w, fib, fstream, ell_fref;
   DevBuf<double> ell_l0;
-  DevBuf<float> ell_il;
+  DevBuf<float> ell_il, fparam;
   DevBuf<ReactDev> reacts;
   DevBuf<double> xs[2], us[2];
   int cur = 0;
@@ -273,12 +277,14 @@ bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE
   if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))
     return false;
   if (regs && b->no_reg_variants) return false;
-  const bool fpar = (v.flags & kFiberPar) != 0;
+  const bool fpar = (v.flags & (kFiberPar | kFiberStream)) != 0;
   if (fpar && (long)v.block * variant_fpl(v.flags) < p.M) return false;
+  // packed 16-bit byte offsets of 12-byte records
+  if ((v.flags & kFiberStream) && 12L * p.N >= 65536) return false;
   if (prec == MF_PREC_FP32C && b->p.fp32_kernel != MF_FP32_KERNEL_AUTO &&
       fpar != (b->p.fp32_kernel == MF_FP32_KERNEL_FIBER))
     return false;
-  return lds_bytes(v, p.N, p.Nb, p.ell_free_entries) <= kMaxLds;
+  return lds_bytes(v, p.N, p.Nb, p.M, p.ell_free_entries) <= kMaxLds;
 }
 
 // Autotuner bucket of an RVE: reactions, register eligibility, lanes needed
@@ -312,14 +318,18 @@ int make_plan(mf_batch *b, int prec) {
       if (!variant_fits(b, prec, v, p, lin)) continue;
       const bool regs = (v.flags & kEllInRegs) != 0;
       const bool fpar = (v.flags & kFiberPar) != 0;
+      const bool fstr = (v.flags & kFiberStream) != 0;
       // reaction modes: linear-only, then one-linear-one-nonlinear, then table
       const int mode_rank = (v.flags & kLinearOnly) ? 0 : (v.flags & kMix2) ? 1 : 2;
-      // register-resident lists with one node per thread; then, for RVEs
-      // with nonlinear fibers, fiber-parallel (fewest threads, fiber slots)
-      // before register lists with several nodes per thread, and after them
-      // for all-linear RVEs (measured order); then the LDS/global-list kernels
-      long key = fpar   ? (lin ? 50000000L : 15000000L) + (long)v.block * 1000 +
-                              (long)variant_fpl(v.flags) * 10 + mode_rank
+      // Measured order.  Register-resident lists with one node per thread.
+      // Then, for RVEs with nonlinear fibers: fiber-parallel with up to 3
+      // fibers per thread, fiber-streamed, fiber-parallel with more fibers,
+      // register lists with several nodes per thread; for all-linear RVEs
+      // the register lists with several nodes per thread come first.  Last
+      // the LDS/global-list kernels.
+      const long fpl = variant_fpl(v.flags);
+      long key = fpar   ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
+                 : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
                  : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
                         : 1000000000L + (long)v.block * 1000000 + (long)v.npt * 1000 - v.flags;
       if (best < 0 || key < best_key) {
@@ -333,7 +343,7 @@ int make_plan(mf_batch *b, int prec) {
       return MF_ERR_UNSUPPORTED;
     }
     b->plan[prec][r] = best;
-    b->lds[prec][r] = lds_bytes(vs[best], p.N, p.Nb, p.ell_free_entries);
+    b->lds[prec][r] = lds_bytes(vs[best], p.N, p.Nb, p.M, p.ell_free_entries);
   }
   return MF_OK;
 }
@@ -342,8 +352,8 @@ int upload_batch(mf_batch *b) {
   const int R = b->R;
   std::vector<RveDev> rd(R);
   std::vector<double> X, mass, l0s, ell_l0;
-  std::vector<int32_t> deg, sb, sD, ew, fib, fst;
-  std::vector<float> il;
+  std::vector<int32_t> deg, sb, sD, ew, fib, fst, efr;
+  std::vector<float> il, fpm;
   std::vector<ReactDev> rx;
   int64_t node_off = 0, ell_off = 0, fib_off = 0;
   int slice_off = 0;
@@ -382,9 +392,11 @@ int upload_batch(mf_batch *b) {
     sD.insert(sD.end(), p.slice_D.begin(), p.slice_D.end());
     ew.insert(ew.end(), p.ell_w.begin(), p.ell_w.end());
     ell_l0.insert(ell_l0.end(), p.ell_l0.begin(), p.ell_l0.end());
-    for (double v : p.ell_l0) il.push_back((float)(1.0 / v));
+    il.insert(il.end(), p.ell_il.begin(), p.ell_il.end());
     fib.insert(fib.end(), p.fib.begin(), p.fib.end());
     fst.insert(fst.end(), p.fstream.begin(), p.fstream.end());
+    efr.insert(efr.end(), p.ell_fref.begin(), p.ell_fref.end());
+    fpm.insert(fpm.end(), p.fparam.begin(), p.fparam.end());
     for (const Reaction &q : p.reactions) {
       ReactDev rdv;
       make_react_dev(q, rdv);
@@ -409,6 +421,8 @@ int upload_batch(mf_batch *b) {
   MF_HIP(b->ell_il.upload(il, s));
   MF_HIP(b->fib.upload(fib, s));
   MF_HIP(b->fstream.upload(fst, s));
+  MF_HIP(b->ell_fref.upload(efr, s));
+  MF_HIP(b->fparam.upload(fpm, s));
   MF_HIP(b->reacts.upload(rx, s));
   // state: x = X, u = 0, V = 1 (constructor, ...Explicit.h:115-118,250-253)
   for (int k = 0; k < 2; ++k) {
@@ -535,6 +549,8 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.ell_il = b->ell_il.p;
   a.fib = b->fib.p;
   a.fstream = b->fstream.p;
+  a.ell_fref = b->ell_fref.p;
+  a.fparam = b->fparam.p;
   a.reacts = b->reacts.p;
   a.x_in = x_in;
   a.u_in = u_in;
@@ -587,6 +603,8 @@ void fill_args(mf_batch *b, SolveArgs &a) {
   a.ell_il = b->ell_il.p;
   a.fib = b->fib.p;
   a.fstream = b->fstream.p;
+  a.ell_fref = b->ell_fref.p;
+  a.fparam = b->fparam.p;
   a.reacts = b->reacts.p;
   a.V = b->V.p;
   a.steps = b->steps.p;
@@ -1125,6 +1143,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
   const mf_solver_params p0 = b->p;
   const std::vector<int> plan0 = b->plan[prec];
   const std::vector<size_t> lds0 = b->lds[prec];
+  const std::map<TuneKey, int> tuned0 = b->tuned;
   b->p.mode = MF_MODE_RAMP;
   b->p.n_steps = steps;
   int timed = 0;
@@ -1167,7 +1186,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
       for (int i : cand) {
         for (int r : L) {
           b->plan[prec][r] = i;
-          b->lds[prec][r] = lds_bytes(vs[i], b->host[r].N, b->host[r].Nb,
+          b->lds[prec][r] = lds_bytes(vs[i], b->host[r].N, b->host[r].Nb, b->host[r].M,
                                       b->host[r].ell_free_entries);
         }
         const int cidx = b->cur;
@@ -1191,14 +1210,54 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
           << key.n256 << ' ' << vs[best].block << ' ' << vs[best].npt << ' ' << vs[best].flags
           << ' ' << best_us << '\n';
   }
+  // Whole-batch check: buckets timed alone can mispredict the batch, whose
+  // launch groups run side by side; keep the tuned buckets only if the full
+  // batch (one solve per RVE, 4x the candidate steps) runs faster with them
+  // than with the plan they replace.
+  bool keep = true;
+  if (err == MF_OK && b->tuned != tuned0) {
+    SolveSet S;
+    S.prec = prec;
+    S.rve.resize(b->R);
+    S.flags.assign(b->R, 0);
+    S.F.assign(9 * (size_t)b->R, 0.0);
+    for (int r = 0; r < b->R; ++r) {
+      S.rve[r] = r;
+      S.F[9 * r] = 1.01;
+      S.F[9 * r + 4] = 1.0;
+      S.F[9 * r + 8] = 1.0;
+    }
+    b->p.n_steps = 4 * steps;
+    auto batch_ms = [&](double &ms) {
+      int e = make_plan(b, prec);
+      const int cidx = b->cur;
+      for (int rep = 0; rep < 2 && e == MF_OK; ++rep) {  // warm-up, then timed
+        e = launch_set(b, S, b->xs[cidx].p, b->us[cidx].p, b->xs[cidx].p, b->us[cidx].p);
+        if (e == MF_OK) e = finish_timing(b);
+      }
+      ms = b->last_ms;
+      return e;
+    };
+    const std::map<TuneKey, int> tuned1 = b->tuned;
+    double ms_old = 0.0, ms_new = 0.0;
+    b->tuned = tuned0;
+    err = batch_ms(ms_old);
+    b->tuned = tuned1;
+    if (err == MF_OK) err = batch_ms(ms_new);
+    keep = err == MF_OK && ms_new < ms_old;
+    if (!keep) b->tuned = tuned0;
+  }
   b->p = p0;
   b->plan[prec] = plan0;
   b->lds[prec] = lds0;
-  if (err != MF_OK) return err;
+  if (err != MF_OK) {
+    b->tuned = tuned0;
+    return err;
+  }
   int e = make_plan(b, prec);
   if (e != MF_OK) return e;
   if ((e = build_identity_tables(b)) != MF_OK) return e;
-  if (cache_path && !added.str().empty()) {
+  if (keep && cache_path && !added.str().empty()) {
     const bool fresh = !std::ifstream(cache_path).good();
     std::ofstream f(cache_path, std::ios::app);
     if (!f) {

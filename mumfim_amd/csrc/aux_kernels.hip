@@ -166,10 +166,19 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kMix2},
     {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},
+    // LDS-list kernels specialised to one reaction mode (large RVEs)
+#define MF_SPEC_ENTRY(B, N, F, MFL) {MF_PREC_FP32C, B, N, (F) | (MFL)},
+    MF_FP32C_SPEC_LIST(MF_SPEC_ENTRY)
+#undef MF_SPEC_ENTRY
     // fiber-parallel: (threads, fibers per thread, free nodes per thread)
 #define MF_FPAR_ENTRY(B, FPL, NPL, FL) {MF_PREC_FP32C, B, NPL, kFiberPar | FL | ((FPL) << kFplShift)},
     MF_FIBER_PAR_LIST(MF_FPAR_ENTRY)
 #undef MF_FPAR_ENTRY
+    // fiber-streamed: (threads, fibers per thread, free nodes per thread)
+#define MF_FSTR_ENTRY(B, FPL, NPL, FL) \
+  {MF_PREC_FP32C, B, NPL, kFiberStream | FL | ((FPL) << kFplShift)},
+    MF_FIBER_STREAM_LIST(MF_FSTR_ENTRY)
+#undef MF_FSTR_ENTRY
 };
 
 const Variant *variants(int *count) {
@@ -177,10 +186,18 @@ const Variant *variants(int *count) {
   return kVariants;
 }
 
-size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries) {
+size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
   const size_t Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
   const size_t NW = v.block / kern::kWave;
   size_t b = 0;
+  if (v.flags & kFiberStream) {
+    // 12-byte displacement and force records (+ zero record), scratch,
+    // boundary state, packed table
+    const size_t nff = (std::max<size_t>((size_t)M + 1, (size_t)N) + 1) & ~size_t(1);
+    b = 12 * (Np + nff) + sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp;
+    b = (b + 15) & ~size_t(15);
+    return b + sizeof(float) * 4 * 3 * kMaxReactions;
+  }
   if (v.prec == MF_PREC_FP64) {
     b += sizeof(double) * 3 * Np * ((v.flags & kDoubleBuf) ? 2 : 1);
   } else if (v.flags & kFiberPar) {

@@ -77,10 +77,13 @@ struct PackedRVE {
   // Sliced ELL incidence lists: slice k holds ranks [64k, 64k+64); entry j of
   // rank r sits at slice_base[k] + 64*j + (r & 63).  Entries of one node are
   // its incident fibers in increasing fiber index (the reference scatter
-  // order).  Packed word = other-end rank | reaction id << 24.
+  // order).  Packed word = other-end rank | reaction id << 24.  Padding
+  // entries (j >= degree) name the node itself with l0 = 1 and 1/l0 = 0, so a
+  // kernel that evaluates them gets a zero force.
   std::vector<int32_t> slice_base, slice_D;
   std::vector<int32_t> ell_w;
   std::vector<double> ell_l0;
+  std::vector<float> ell_il;           // FP32 1/l0 (0 at padding)
   int ell_free_entries = 0;            // entries of slices that hold free nodes
   int max_free_deg = 0;                // largest degree of a free node
   int n_split = 0;                     // free nodes with degree > kLaneDegree
@@ -90,6 +93,15 @@ struct PackedRVE {
   // consecutive fibers mostly have lane-consecutive a; fibers with two
   // boundary ends come last.
   std::vector<int32_t> fstream;
+  // Per ELL entry: the entry's fiber as a byte offset of its 12-byte force
+  // record in the fiber-streamed kernel (stream position * 12), bit 31 set
+  // where the node is the fiber's end a (it receives -force).  Padding and
+  // boundary-node entries name the zero record at position M.
+  std::vector<int32_t> ell_fref;
+  // Per fiber in stream order, 4 floats: the reference vector X_b - X_a
+  // rounded once to FP32, and the law coefficient (linear: E A / (2 l0^3);
+  // nonlinear: -1/l0; 0 for fibers with two boundary ends).
+  std::vector<float> fparam;
   std::vector<int32_t> fib;            // 2M kernel ranks, input fiber order
   std::vector<double> l0;              // M
   std::vector<int32_t> rid;            // M (after the reaction policy)

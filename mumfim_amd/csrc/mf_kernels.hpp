@@ -24,6 +24,8 @@ struct SolveArgs {
   const float *ell_il;      // FP32 kernels: 1/l0
   const int32_t *fib;       // 2 per fiber (kernel ranks)
   const int32_t *fstream;   // 4 per fiber: ranks a, b and their ELL entries
+  const int32_t *ell_fref;  // per ELL entry: force-record byte offset | sign bit
+  const float *fparam;      // 4 per fiber: reference vector, law coefficient
   const ReactDev *reacts;
   const double *x_in, *u_in;
   double *x_out, *u_out;
@@ -57,10 +59,12 @@ enum : int32_t {
   kEnergy = 16,     // FP64 with energy diagnostics
   kMix2 = 32,       // at most one linear and one nonlinear reaction (FP32c registers)
   kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
+  kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
 };
 // Fibers per thread of a fiber-parallel variant (flags bits 8-11).
 constexpr int kFplShift = 8;
 inline int variant_fpl(int32_t flags) { return (flags >> kFplShift) & 15; }
+
 
 
 struct Variant {
@@ -72,8 +76,8 @@ struct Variant {
 
 // All compiled variants (the planner picks from this list).
 const Variant *variants(int *count);
-// LDS bytes a variant needs for one RVE.
-size_t lds_bytes(const Variant &v, int N, int Nb, int ell_free_entries);
+// LDS bytes a variant needs for one RVE (N nodes, Nb boundary, M fibers).
+size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries);
 // Largest dynamic LDS a workgroup may request.
 constexpr size_t kMaxLds = 160 * 1024;
 

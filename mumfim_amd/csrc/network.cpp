@@ -407,14 +407,21 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
     if (k + 1 == n_free_slices) p.ell_free_entries = total;
   }
   if (n_free_slices == 0) p.ell_free_entries = 0;
-  p.ell_w.assign(total, -1);
+  p.ell_w.assign(total, 0);
   p.ell_l0.assign(total, 1.0);
+  p.ell_il.assign(total, 0.0f);
+  for (int k = 0; k < n_slices; ++k)  // padding: the node itself
+    for (int j = 0; j < p.slice_D[k]; ++j)
+      for (int lane = 0; lane < kSlice; ++lane)
+        p.ell_w[(size_t)p.slice_base[k] + (size_t)j * kSlice + lane] =
+            k * kSlice + lane < N ? k * kSlice + lane : 0;
   for (int r = 0; r < N; ++r) {
     int k = r / kSlice, lane = r % kSlice;
     for (int j = 0; j < p.deg[r]; ++j) {
       size_t at = (size_t)p.slice_base[k] + (size_t)j * kSlice + lane;
       p.ell_w[at] = csr_w[row[r] + j];
       p.ell_l0[at] = csr_l0[row[r] + j];
+      p.ell_il[at] = (float)(1.0 / csr_l0[row[r] + j]);
     }
   }
   // fiber stream: ELL entry of every free fiber end, then fibers in entry
@@ -446,6 +453,25 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
       }
   for (int e = 0; e < M; ++e)
     if (!taken[e]) p.fstream.insert(p.fstream.end(), {p.fib[2 * e], p.fib[2 * e + 1], -1, -1});
+  p.fparam.assign(4 * (size_t)M, 0.0f);
+  for (int e = 0; e < M; ++e) {
+    const int ra = p.fstream[4 * (size_t)e], rb = p.fstream[4 * (size_t)e + 1];
+    const int sa = p.fstream[4 * (size_t)e + 2];
+    float *q = &p.fparam[4 * (size_t)e];
+    for (int d = 0; d < 3; ++d) q[d] = (float)(p.X[3 * rb + d] - p.X[3 * ra + d]);
+    if (sa >= 0) {
+      const Reaction &rq = p.reactions[(uint32_t)p.ell_w[sa] >> 24];
+      const double l0 = p.ell_l0[sa];
+      q[3] = rq.type == MF_REACTION_NONLINEAR ? -p.ell_il[sa]
+                                              : (float)(0.5 * rq.E * rq.area() / (l0 * l0 * l0));
+    }
+  }
+  p.ell_fref.assign(total, 12 * M);
+  for (int e = 0; e < M; ++e) {
+    const int sa = p.fstream[4 * (size_t)e + 2], sb = p.fstream[4 * (size_t)e + 3];
+    if (sa >= 0) p.ell_fref[sa] = (int32_t)(12u * (uint32_t)e | 0x80000000u);
+    if (sb >= 0) p.ell_fref[sb] = 12 * e;
+  }
   return MF_OK;
 }
 

@@ -7,6 +7,11 @@
 namespace mf {
 namespace kern {
 
+// Reaction modes: RM_TABLE looks each fiber's law up in the packed table;
+// RM_LIN: the RVE has one linear reaction; RM_MIX2: at most one linear and
+// one nonlinear reaction, toe range within the expm1 polynomial's bound.
+enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
+
 // ============================================================================
 // FP32 + compensated kernel
 // ============================================================================
@@ -23,7 +28,10 @@ __device__ inline void two_sum(float a, float b, float &s, float &e) {
 // displacement difference delta = u_w - u_v, current vector d = d0 + delta
 // and l^2 - l0^2 = (d0 + d) . delta, which keeps full relative precision at
 // small strain (the direct form d.d - l0^2 loses eps/strain).
-template <int B, int NPT, int FL>
+// RM < 0: the reaction mode is read per RVE and each incidence entry is
+// evaluated in a loop; RM = RM_LIN / RM_MIX2: mode-specialised straight-line
+// gathers (variants for large RVEs).
+template <int B, int NPT, int FL, int RM = -1>
 __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   constexpr bool kEll = (FL & kEllInLds) != 0;
   constexpr bool kDbuf = (FL & kDoubleBuf) != 0;
@@ -78,7 +86,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 
   float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
   // slice width and base are wave-uniform (scalar registers)
-  int degr[NPT], Dr[NPT], sbase[NPT];
+  int Dr[NPT], sbase[NPT];
   // initial displacement: relax adds the affine increment (F - I) x to u,
   // ramp keeps u and sets the boundary increment aside.
   auto init_node = [&](int r, double (&u)[3]) {
@@ -124,12 +132,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       ul[s][d] = (float)(u[d] - (double)uh[s][d]);
       vr[s][d] = 0.0f;
     }
-    degr[s] = 0;
     imr[s] = 0.0f;
-    if (r < Nf) {
-      imr[s] = (float)(1.0 / a.mass[nb + r]);
-      degr[s] = a.deg[nb + r];
-    }
+    if (r < Nf) imr[s] = (float)(1.0 / a.mass[nb + r]);
     // lane 0 of the wave names the slice; waves past Nf never gather
     const int k = r / kWave;
     Dr[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
@@ -144,37 +148,107 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   const float inv_c = (float)(1.0 / R.c_sound);
   const float xq_inv = R.xq_inv;
   int cur = 0;
+  // internal force of a slot's node.  ELL padding names the node itself
+  // with 1/l0 = 0 (zero force), so every entry of the slice width is
+  // evaluated unmasked.  In the mode-specialised variants the wave-uniform
+  // width selects straight-line batches of 4 entries whose list and
+  // position loads issue together.
   auto node_force = [&](int s, int buf, float (&f)[3]) {
     const float4 *P = pos + (kDbuf ? buf * Np : 0);
     const int4 Xv = Xs[s * B + tid];  // own reference coordinate
+    const int base = sbase[s] + lane;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    const int D = Dr[s];
-    for (int j = 0; j < D; ++j) {
-      if (j < degr[s]) {
-        const int idx = sbase[s] + lane + j * kWave;
-        const int32_t wv = Ew[idx];
-        const float il = Eil[idx];
+    auto entry = [&](auto MC, int32_t wv, float il, const float4 &q, const int4 &xq) {
+      constexpr int MODE = decltype(MC)::value;
+      // reference vector: exact integer difference, one rounding to FP32
+      const float ex = (float)(xq.x - Xv.x) * xq_inv;
+      const float ey = (float)(xq.y - Xv.y) * xq_inv;
+      const float ez = (float)(xq.z - Xv.z) * xq_inv;
+      const float gx = (q.x - uh[s][0]) - ul[s][0];
+      const float gy = (q.y - uh[s][1]) - ul[s][1];
+      const float gz = (q.z - uh[s][2]) - ul[s][2];
+      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+      // l^2 - l0^2 = (d0 + d) . delta
+      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float e2 = sl * (il * il);
+      const unsigned rid = (uint32_t)wv >> 24;
+      float t;
+      if constexpr (MODE == RM_LIN)
+        t = lin_EA_half * il * e2;
+      else if constexpr (MODE == RM_MIX2)
+        t = react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il);
+      else
+        t = lin_only ? lin_EA_half * il * e2
+            : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
+                     : react32_tab(rtab, rid, e2, il);
+      fx -= dx * t;
+      fy -= dy * t;
+      fz -= dz * t;
+    };
+    auto batch = [&](auto MC, auto J0C, auto KC) {
+      constexpr int J0 = decltype(J0C)::value, K = decltype(KC)::value;
+      int32_t wv[K];
+      float il[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int idx = base + (J0 + j) * kWave;
+        wv[j] = Ew[idx];
+        il[j] = Eil[idx];
+      }
+      float4 q[K];
+      int4 xq[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int w = wv[j] & 0xFFFFFF;
+        q[j] = P[w];
+        xq[j] = Xs[w];
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) entry(MC, wv[j], il[j], q[j], xq[j]);
+    };
+    auto body = [&](auto MC, auto KC) {
+      constexpr int K = decltype(KC)::value;
+      using I = std::integral_constant<int, 0>;
+      batch(MC, I{}, std::integral_constant<int, (K < 4 ? K : 4)>{});
+      if constexpr (K > 4)
+        batch(MC, std::integral_constant<int, 4>{}, std::integral_constant<int, (K < 8 ? K - 4 : 4)>{});
+      if constexpr (K > 8)
+        batch(MC, std::integral_constant<int, 8>{}, std::integral_constant<int, K - 8>{});
+    };
+    using MC = std::integral_constant<int, RM>;
+    // rolling prefetch: entry j+1's list word and 1/l0 load while entry j
+    // is evaluated
+    auto loop = [&]() {
+      const int D = Dr[s];
+      if (D <= 0) return;
+      int32_t wv_n = Ew[base];
+      float il_n = Eil[base];
+      for (int j = 0; j < D; ++j) {
+        const int32_t wv = wv_n;
+        const float il = il_n;
+        if (j + 1 < D) {
+          wv_n = Ew[base + (j + 1) * kWave];
+          il_n = Eil[base + (j + 1) * kWave];
+        }
         const int w = wv & 0xFFFFFF;
-        const float4 q = P[w];
-        const int4 xq = Xs[w];
-        // reference vector: exact integer difference, one rounding to FP32
-        const float ex = (float)(xq.x - Xv.x) * xq_inv;
-        const float ey = (float)(xq.y - Xv.y) * xq_inv;
-        const float ez = (float)(xq.z - Xv.z) * xq_inv;
-        const float gx = (q.x - uh[s][0]) - ul[s][0];
-        const float gy = (q.y - uh[s][1]) - ul[s][1];
-        const float gz = (q.z - uh[s][2]) - ul[s][2];
-        const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-        // l^2 - l0^2 = (d0 + d) . delta
-        const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
-        const float e2 = sl * (il * il);
-        const unsigned rid = (uint32_t)wv >> 24;
-        const float t = lin_only ? lin_EA_half * il * e2
-                        : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
-                                 : react32_tab(rtab, rid, e2, il);
-        fx -= dx * t;
-        fy -= dy * t;
-        fz -= dz * t;
+        entry(MC{}, wv, il, P[w], Xs[w]);
+      }
+    };
+    if constexpr (RM < 0) {
+      loop();
+    } else {
+      switch (Dr[s]) {
+#define MF_DEG_CASE(k) \
+  case k:              \
+    body(MC{}, std::integral_constant<int, k>{}); \
+    break;
+        MF_DEG_CASE(1) MF_DEG_CASE(2) MF_DEG_CASE(3) MF_DEG_CASE(4)
+        MF_DEG_CASE(5) MF_DEG_CASE(6) MF_DEG_CASE(7) MF_DEG_CASE(8)
+        MF_DEG_CASE(9) MF_DEG_CASE(10) MF_DEG_CASE(11) MF_DEG_CASE(12)
+#undef MF_DEG_CASE
+        default:
+          loop();
+          break;
       }
     }
     f[0] = fx;
@@ -421,12 +495,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 // ============================================================================
 constexpr int kLaneDeg = kLaneDegree;
 
-// Reaction modes: RM_TABLE looks each fiber's law up in the packed table;
-// RM_LIN (one linear reaction) keeps EA/(2 l0^3) per fiber; RM_MIX2 (at most
-// one linear and one nonlinear reaction, toe range within the expm1
-// polynomial's bound) keeps EA/(2 l0^3) for linear fibers and -1/l0 for
-// nonlinear ones, with the nonlinear law's constants uniform.
-enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
+// Reaction modes (RM_* above) of the register kernel: RM_LIN keeps
+// EA/(2 l0^3) per fiber; RM_MIX2 keeps EA/(2 l0^3) for linear fibers and
+// -1/l0 for nonlinear ones, with the nonlinear law's constants uniform.
 
 template <int B, int NPT, int RM>
 __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
@@ -1341,6 +1412,421 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
   }
 }
 
+// ============================================================================
+// FP32c fiber-streamed kernel (large RVEs)
+// ============================================================================
+// Each fiber is evaluated once per step by the lane that owns it (endpoint
+// byte offsets packed in one register, reference vector and law coefficient
+// in registers; streaming them from global memory each step measured
+// slower) and its force t d is written to a 12-byte record in LDS, in
+// fiber-stream order (lane-consecutive stores).  Each free node then sums
+// +-force over its incidence entries, whose record offsets and signs stream
+// from the ELL in global memory with a rolling prefetch.  Positions and
+// forces are 12-byte records, so LDS holds about 37 bytes per node plus the
+// boundary state: RVEs up to ~3700 nodes.  Arithmetic per fiber is the
+// register kernel's (both ends as FP32-rounded displacements, exactly rounded
+// reference vectors).
+struct F3 {
+  float x, y, z;
+};
+
+template <int B, int FPL, int NPL, int RM>
+__global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
+  constexpr bool LIN = RM == RM_LIN;
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int nff = ((max(M + 1, N)) + 1) & ~1;  // force records (+ zero record)
+  const int64_t nb = R.node_off;
+
+  size_t off = 0;
+  F3 *pos = reinterpret_cast<F3 *>(smem);  // [Np] displacement hi
+  off += sizeof(F3) * Np;
+  F3 *ffr = reinterpret_cast<F3 *>(smem + off);  // [nff] fiber forces t d
+  off += sizeof(F3) * nff;
+  double *red = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  off += sizeof(double) * 6 * Nbp;
+  off = (off + 15) & ~size_t(15);
+  float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed table (finalize)
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const float *gil = a.ell_il + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  const int32_t *gfr = a.ell_fref + R.ell_off;
+  const double *X = a.X + 3 * nb;
+  if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
+  const double EA_half = 0.5 * a.reacts[R.react_off].E * a.reacts[R.react_off].A;
+  ReactDev nlq;
+  if (RM == RM_MIX2) {
+    const int k = (R.n_react > 1 && a.reacts[R.react_off + 1].type == MF_REACTION_NONLINEAR) ? 1 : 0;
+    nlq = a.reacts[R.react_off + k];
+  }
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  auto init_u = [&](int r, double (&u)[3], double (&inc)[3]) {
+    const int64_t g = nb + r;
+    double xp[3], up[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+      u[k] = ramp ? up[k] : inc[k] + up[k];
+    }
+  };
+  for (int r = tid; r < N; r += B) {
+    double u[3], inc[3];
+    init_u(r, u, inc);
+    pos[r] = F3{(float)u[0], (float)u[1], (float)u[2]};
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = u[k];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  }
+  for (int i = tid; i < nff; i += B) ffr[i] = F3{0.0f, 0.0f, 0.0f};
+
+  // free nodes: r = s * B + tid
+  float uh[NPL][3], ul[NPL][3], vr[NPL][3], imr[NPL];
+  int Dn[NPL], sbn[NPL];
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    imr[s] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uh[s][d] = ul[s][d] = vr[s][d] = 0.0f;
+    if (r < Nf) {
+      double u[3], inc[3];
+      init_u(r, u, inc);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        uh[s][d] = (float)u[d];
+        ul[s][d] = (float)(u[d] - (double)uh[s][d]);
+      }
+      imr[s] = (float)(1.0 / a.mass[nb + r]);
+    }
+    const int k = r / kWave;
+    Dn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
+    sbn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
+  }
+
+  // fibers: e = s * B + tid in stream order; endpoint byte offsets a | b << 16
+  int pab[FPL];
+  float d0x[FPL], d0y[FPL], d0z[FPL], cf[FPL];
+  bool live[FPL];
+  const int32_t *fs = a.fstream + 4 * R.fib_off;
+  const float4 *fp = reinterpret_cast<const float4 *>(a.fparam) + R.fib_off;
+#pragma unroll
+  for (int s = 0; s < FPL; ++s) {
+    const int e = s * B + tid;
+    live[s] = __builtin_amdgcn_readfirstlane(s * B + (tid & ~(kWave - 1))) < M;
+    pab[s] = 0;
+    d0x[s] = d0y[s] = d0z[s] = cf[s] = 0.0f;
+    if (e < M) {
+      const int4 q = *reinterpret_cast<const int4 *>(fs + 4 * e);
+      pab[s] = (q.x * (int)sizeof(F3)) | ((q.y * (int)sizeof(F3)) << 16);
+      const float4 c = fp[e];
+      d0x[s] = c.x;
+      d0y[s] = c.y;
+      d0z[s] = c.z;
+      cf[s] = c.w;
+    }
+  }
+  __syncthreads();
+
+  const float inv_c = (float)(1.0 / R.c_sound);
+  const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
+
+  // forces t d of the owned fibers into their records; squared current
+  // length minimum over the owned fibers when asked
+  auto fiber_phase = [&](bool want_min) {
+    const unsigned char *P = reinterpret_cast<const unsigned char *>(pos);
+    float lmin = 3.0e38f;
+#pragma unroll
+    for (int s = 0; s < FPL; ++s) {
+      if (!live[s]) continue;
+      const float px = d0x[s], py = d0y[s], pz = d0z[s], pc = cf[s];
+      const F3 qa = *reinterpret_cast<const F3 *>(P + (pab[s] & 0xFFFF));
+      const F3 qb = *reinterpret_cast<const F3 *>(P + ((uint32_t)pab[s] >> 16));
+      const float gx = qb.x - qa.x, gy = qb.y - qa.y, gz = qb.z - qa.z;
+      const float dx = px + gx, dy = py + gy, dz = pz + gz;
+      // l^2 - l0^2 = (d0 + d) . delta
+      const float sq = (px + dx) * gx + (py + dy) * gy + (pz + dz) * gz;
+      float t;
+      if (RM == RM_LIN) {
+        t = pc * sq;
+      } else {
+        const float c = pc;
+        const float il = -c;
+        const float e2 = sq * (il * il);
+        const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
+        const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
+        const float tt = nlq.alpha * il * expm1_poly(x);
+        const float lam2 = 1.0f + e2;
+        const float ilam = __frsqrt_rn(lam2);
+        const float lam = lam2 * ilam;
+        const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
+        const float bb = hi ? nlq.bt : nlq.bc;
+        const float te = (K * (lam - L0) + bb) * il * ilam;
+        t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
+      }
+      const int e = s * B + tid;
+      if (e < M) {
+        if (want_min) lmin = fminf(lmin, dx * dx + dy * dy + dz * dz);
+        ffr[e] = F3{dx * t, dy * t, dz * t};
+      }
+    }
+    return lmin;
+  };
+  // internal force of a free node: -force of fibers where it is end a,
+  // +force where it is end b; references prefetched one entry ahead
+  auto node_force = [&](int s, float (&f)[3]) {
+    const unsigned char *Fb = reinterpret_cast<const unsigned char *>(ffr);
+    const int32_t *E = gfr + sbn[s] + lane;
+    const int D = Dn[s];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (D > 0) {
+      int32_t nx = E[0];
+      for (int j = 0; j < D; ++j) {
+        const int32_t fr = nx;
+        if (j + 1 < D) nx = E[(j + 1) * kWave];
+        const F3 v = *reinterpret_cast<const F3 *>(Fb + (fr & 0x7FFFFFFF));
+        const float sg = __int_as_float((fr & (int32_t)0x80000000) | 0x3F800000);
+        fx = fmaf(sg, v.x, fx);
+        fy = fmaf(sg, v.y, fy);
+        fz = fmaf(sg, v.z, fz);
+      }
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  auto wave_min_to = [&](float v, double *slot) {
+    double m = wave_min((double)v);
+    if (lane == 0) slot[(tid / kWave) * kRedStride] = m;
+  };
+  auto red_min = [&](const double *slot) {
+    double m = slot[0];
+#pragma unroll
+    for (int k = 1; k < NW; ++k) m = fmin(m, slot[k * kRedStride]);
+    return m;
+  };
+
+  // The closing half kick of step k and the opening half kick of step k+1
+  // use the same acceleration: v_{k+1/2} = v_{k-1/2} + (h_k + h_{k+1}) a_k,
+  // applied as soon as a_k and h_{k+1} are known (v_{-1/2} = 0, h_{-1} = 0).
+  const float crit_half = (float)(a.crit * 0.5);
+  float h_prev = 0.0f;
+  {
+    const float lm = fiber_phase(true);
+    wave_min_to(lm, red);
+    __syncthreads();
+    const float h = crit_half * (sqrtf((float)red_min(red)) * inv_c);
+#pragma unroll
+    for (int s = 0; s < NPL; ++s)
+      if (s * B + tid < Nf) {
+        float f[3];
+        node_force(s, f);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + h * (-f[d] * imr[s] - eta * vr[s][d]);
+      }
+    h_prev = h;
+  }
+
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  for (;;) {
+    const float h2 = 2.0f * h_prev;
+    // drift: u (hi, lo) += 2h v with Fast2Sum, exact when |u_hi| >= |increment|;
+    // otherwise u is within a few increments of zero and the uncompensated
+    // rounding is below eps * |increment|.
+#pragma unroll
+    for (int s = 0; s < NPL; ++s) {
+      const int r = s * B + tid;
+      if (r < Nf) {
+        float pc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float y = ul[s][d] + h2 * vr[s][d];
+          const float sh = uh[s][d] + y;
+          ul[s][d] = y - (sh - uh[s][d]);
+          uh[s][d] = sh;
+          pc[d] = sh;
+        }
+        pos[r] = F3{pc[0], pc[1], pc[2]};
+      }
+    }
+    if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
+      for (int b = tid; b < Nb; b += B)
+        pos[Nf + b] = F3{(float)(bst[b] + alpha * bst[3 * Nbp + b]),
+                         (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
+                         (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b])};
+    }
+    __syncthreads();  // positions published; every force read of the last step done
+    const bool refresh = (step % freq) == 0;
+    const float lm = fiber_phase(refresh);
+    if (refresh) wave_min_to(lm, red);
+    __syncthreads();  // fiber forces published
+    float h = h_prev;
+    if (refresh) h = crit_half * (sqrtf((float)red_min(red)) * inv_c);
+    const float hk = h_prev + h;
+    double rsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NPL; ++s)
+      if (s * B + tid < Nf) {
+        float f[3];
+        node_force(s, f);
+        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
+      }
+    if (refresh) {
+      // residual in slots 1 of the scratch (slots 0 are read above)
+      const double ws = wave_sum(rsum);
+      if (lane == 0) red[(tid / kWave) * kRedStride + 1] = ws;
+      __syncthreads();
+      double sum = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) sum += red[k * kRedStride + 1];
+      residual = sqrt(sum);
+    }
+    h_prev = h;
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // finalize: low parts into the force records, boundary gathers in FP32
+  // with exactly rounded reference vectors, stress in FP64
+  F3 *Ph = pos;
+  F3 *Pl = ffr;
+  __syncthreads();  // every force and scratch read is done
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) Pl[r] = F3{ul[s][0], ul[s][1], ul[s][2]};
+  }
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    double ub[3];
+    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const F3 hb{(float)ub[0], (float)ub[1], (float)ub[2]};
+    Ph[Nf + b] = hb;
+    Pl[Nf + b] = F3{(float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
+                    (float)(ub[2] - (double)hb.z)};
+  }
+  __syncthreads();
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const int flags = a.solve_flags[sid];
+  for (int b = tid; b < Nb; b += B) {
+    const int rb = Nf + b;
+    const int64_t gr = nb + rb;
+    double ub[3], xb[3];
+    for (int d = 0; d < 3; ++d) {
+      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+      xb[d] = X[3 * rb + d] + ub[d];
+    }
+    const F3 hb = Ph[rb], lb = Pl[rb];
+    const int k = rb / kWave;
+    const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
+    const int dg = a.deg[gr];
+    float f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const int w = wv & 0xFFFFFF;
+      const F3 qh = Ph[w], ql = Pl[w];
+      const float ex = (float)(X[3 * w] - X[3 * rb]);
+      const float ey = (float)(X[3 * w + 1] - X[3 * rb + 1]);
+      const float ez = (float)(X[3 * w + 2] - X[3 * rb + 2]);
+      const float gx = (qh.x - hb.x) + (ql.x - lb.x);
+      const float gy = (qh.y - hb.y) + (ql.y - lb.y);
+      const float gz = (qh.z - hb.z) + (ql.z - lb.z);
+      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      float t;
+      if (LIN) {
+        const double l0 = gl0[idx];
+        t = (float)(EA_half / (l0 * l0 * l0)) * sq;
+      } else {
+        const float il = gil[idx];
+        t = react32_tab(rtab, (uint32_t)wv >> 24, sq * (il * il), il);
+      }
+      f1 -= dx * t;
+      f2 -= dy * t;
+      f3 -= dz * t;
+    }
+    const double g1 = f1, g2 = f2, g3 = f3;
+    st[0] += xb[0] * g1;
+    st[1] += xb[1] * g2;
+    st[2] += xb[2] * g3;
+    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
+    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
+    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * gr + d] = ub[d];
+  }
+#pragma unroll
+  for (int s = 0; s < NPL; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) {
+      const int64_t g = nb + r;
+      for (int d = 0; d < 3; ++d) {
+        const double u = (double)uh[s][d] + (double)ul[s][d];
+        if (flags & kWriteX) a.x_out[3 * g + d] = X[3 * r + d] + u;
+        if (flags & kWriteU) a.u_out[3 * g + d] = u;
+      }
+    }
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !isfinite(v);
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
 }  // namespace kern
 
 hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
@@ -1358,6 +1844,18 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
+#define MF_SPEC_LAUNCH(B, N, F, MFL)                                          \
+  if (v.block == B && v.npt == N && v.flags == ((F) | (MFL))) {               \
+    auto k = kern::solve32c<B, N, F, (MFL) == kLinearOnly ? kern::RM_LIN : kern::RM_MIX2>; \
+    if (lds > 64 * 1024)                                                      \
+      (void)hipFuncSetAttribute((const void *)k,                              \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                (int)kMaxLds);                                \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                \
+    return hipGetLastError();                                                 \
+  }
+  MF_FP32C_SPEC_LIST(MF_SPEC_LAUNCH)
+#undef MF_SPEC_LAUNCH
 #define MF_REG_LAUNCH(B, N, RMODE, FLAG)                                       \
   if (v.npt == N && v.block == B && (v.flags & kEllInRegs) &&                  \
       (v.flags & (kLinearOnly | kMix2)) == FLAG) {                             \
@@ -1404,6 +1902,20 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_FIBER_PAR_LIST(MF_FPAR_LAUNCH)
 #undef MF_FPAR_LAUNCH
+#define MF_FSTR_LAUNCH(B, FPL, NPL, FL)                                       \
+  if (v.block == B && v.npt == NPL &&                                         \
+      v.flags == (kFiberStream | (FL) | ((FPL) << kFplShift))) {              \
+    auto k = kern::solve32s<B, FPL, NPL,                                      \
+                            (FL) == kLinearOnly ? kern::RM_LIN : kern::RM_MIX2>; \
+    if (lds > 64 * 1024)                                                      \
+      (void)hipFuncSetAttribute((const void *)k,                              \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                (int)kMaxLds);                                \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                \
+    return hipGetLastError();                                                 \
+  }
+  MF_FIBER_STREAM_LIST(MF_FSTR_LAUNCH)
+#undef MF_FSTR_LAUNCH
   return hipErrorInvalidValue;
 }
 

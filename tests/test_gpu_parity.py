@@ -292,18 +292,18 @@ def test_register_kernel_slots_agree_bitwise():
         nets = [mf.generate_network(seed=60 + i, n_nodes=n, nonlinear_fraction=nl)
                 for i, n in enumerate([700, 1000])]
         F = strain_F(6, 2, 0.05)
-        p = ramp_params(n_steps=600, precision=mf.MF_PREC_FP32C,
-                        reaction_policy=mf.MF_REACT_PER_FIBER)
+        node = dict(n_steps=600, precision=mf.MF_PREC_FP32C,
+                    reaction_policy=mf.MF_REACT_PER_FIBER, fp32_kernel=mf.MF_FP32_KERNEL_NODE)
+        p = ramp_params(**node)
         g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
         _, plan = g.plan()
         ref = g.run(F)
         assert np.all(plan[:, 1] == 1), plan  # one node per thread by default
-        forced = mf.BatchedFiberRVEAnalysisExplicit(
-            nets, ramp_params(n_steps=600, precision=mf.MF_PREC_FP32C,
-                              reaction_policy=mf.MF_REACT_PER_FIBER, block_size=512))
+        assert np.all(plan[:, 2] & 4), plan  # register-resident lists
+        forced = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(block_size=512, **node))
         _, fplan = forced.plan()
         say(f"  nonlinear {nl}: default {plan[:, :2].tolist()} forced {fplan[:, :2].tolist()}")
-        assert np.all(fplan[:, 1] >= 2)
+        assert np.all(fplan[:, 1] >= 2) and np.all(fplan[:, 2] & 4), fplan
         # per-node arithmetic is the same: the committed state agrees bitwise;
         # the FP64 stress sum runs over a different number of threads
         sf = forced.run(F)
@@ -317,15 +317,17 @@ def test_register_kernel_slots_agree_bitwise():
         assert err <= FP32C_TOL
 
 
-def test_fiber_parallel_kernel_matches_oracle():
-    """The fiber-parallel FP32c kernel (each fiber evaluated once, nodes sum
-    the published end forces) in its linear, one-linear-one-nonlinear and
-    reaction-table modes: within the FP32c bar of the oracle, and close to
-    the node-centric kernels on the same solves (the two round the per-fiber
-    force contributions differently)."""
-    say("case: fiber-parallel kernel, three reaction modes")
+def test_fiber_kernels_match_oracle():
+    """The two fiber-evaluating FP32c kernel families, each fiber evaluated
+    once per step: fiber-parallel (end forces into per-node slots; linear,
+    one-linear-one-nonlinear and reaction-table modes) and fiber-streamed
+    (per-fiber force records, nodes gather by reference; large RVEs).  Within
+    the FP32c bar of the oracle, and close to the node-centric kernels on the
+    same solves (the families round the per-fiber contributions differently)."""
+    say("case: fiber kernels, three reaction modes, 480-2400 nodes")
     nets = [mf.generate_network(seed=80 + i, n_nodes=n, nonlinear_fraction=nl)
-            for i, (n, nl) in enumerate([(480, 0.0), (1150, 0.5), (1400, 0.5)])]
+            for i, (n, nl) in enumerate([(480, 0.0), (1150, 0.5), (1400, 0.5),
+                                         (2400, 0.5), (2400, 0.0)])]
     # a second nonlinear reaction (B = 2) puts the RVE in reaction-table mode
     t = mf.generate_network(seed=90, n_nodes=480, nonlinear_fraction=0.5)
     t.reactions.append(dataclasses.replace(t.reactions[1], B=2.0))
@@ -337,15 +339,17 @@ def test_fiber_parallel_kernel_matches_oracle():
         nets, ramp_params(precision=mf.MF_PREC_FP32C, fp32_kernel=mf.MF_FP32_KERNEL_FIBER, **common))
     _, plan = fib.plan()
     say(f"  fiber plan {plan[:, :3].tolist()}")
-    assert np.all(plan[:, 2] & 64), plan
+    fam = [int(f) & (64 | 128) for f in plan[:, 2]]
+    assert all(fam), plan
+    assert 64 in fam and 128 in fam, fam  # both families exercised
     modes = [int(f) & (8 | 32) for f in plan[:, 2]]
-    assert modes[0] == 8 and modes[1] == 32 and modes[2] == 32 and modes[3] == 0, modes
+    assert modes == [8, 32, 32, 32, 8, 0], modes
     s_f, st = fib.run(F, return_status=True)
     assert (st == mf.MF_STATUS_OK).all()
     node = mf.BatchedFiberRVEAnalysisExplicit(
         nets, ramp_params(precision=mf.MF_PREC_FP32C, fp32_kernel=mf.MF_FP32_KERNEL_NODE, **common))
     _, nplan = node.plan()
-    assert not np.any(nplan[:, 2] & 64), nplan
+    assert not np.any(nplan[:, 2] & (64 | 128)), nplan
     s_n = node.run(F)
     s_o = Oracle(nets, ramp_params(**common)).run(F)
     e_f, e_n, e_fn = rel_err(s_f, s_o), rel_err(s_n, s_o), rel_err(s_f, s_n)
