@@ -499,6 +499,33 @@ int create_from_networks(std::vector<Network> &nets, const mf_solver_params *par
   return MF_OK;
 }
 
+// Launch order of n solves (solve i runs RVE rve(i)): grouped by kernel
+// variant, groups in decreasing order of their largest solve (fibers), and
+// largest solves first inside a group.  Longest workgroups start first, so
+// they do not trail the batch; a small group of a wider variant also gets
+// empty CUs before the big groups fill them (a CU running the common variant
+// may have no room left for its wider workgroups).
+template <class RveOf>
+std::vector<int32_t> launch_order(const mf_batch *b, int prec, int n, RveOf rve) {
+  std::map<int, int> max_m;  // variant -> largest fiber count
+  for (int i = 0; i < n; ++i) {
+    const int r = rve(i), v = b->plan[prec][r];
+    max_m[v] = std::max(max_m[v], b->host[r].M);
+  }
+  std::vector<int32_t> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    const int ra = rve(a), rc = rve(c);
+    const int va = b->plan[prec][ra], vc = b->plan[prec][rc];
+    if (va != vc) {
+      const int ma = max_m.at(va), mc = max_m.at(vc);
+      return ma != mc ? ma > mc : va < vc;
+    }
+    return b->host[ra].M > b->host[rc].M;
+  });
+  return order;
+}
+
 // Launches one set of solves.  solve_* arrays live on the host and are
 // uploaded; sigma/status/steps land in the device scratch.
 struct SolveSet {
@@ -519,15 +546,8 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   MF_HIP(b->sigma.ensure(6 * (size_t)n));
   MF_HIP(b->status.ensure(n));
   MF_HIP(b->steps.ensure(n));
-  // group by variant, largest RVEs first inside a group
   const int prec = S.prec;
-  std::vector<int32_t> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-    int va = b->plan[prec][S.rve[a]], vc = b->plan[prec][S.rve[c]];
-    if (va != vc) return va < vc;
-    return b->host[S.rve[a]].M > b->host[S.rve[c]].M;
-  });
+  const std::vector<int32_t> order = launch_order(b, prec, n, [&](int i) { return S.rve[i]; });
   MF_HIP(hipMemcpyAsync(b->solve_rve.p, S.rve.data(), sizeof(int32_t) * n,
                         hipMemcpyHostToDevice, s));
   MF_HIP(hipMemcpyAsync(b->solve_flags.p, S.flags.data(), sizeof(int32_t) * n,
@@ -632,12 +652,7 @@ int build_identity_tables(mf_batch *b) {
   MF_HIP(b->id_F.alloc(9 * (size_t)R));
   MF_HIP(b->steps.ensure(R));
   for (int prec = 0; prec < 2; ++prec) {
-    std::vector<int32_t> order(ids);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-      int va = b->plan[prec][a], vc = b->plan[prec][c];
-      if (va != vc) return va < vc;
-      return b->host[a].M > b->host[c].M;
-    });
+    const std::vector<int32_t> order = launch_order(b, prec, R, [](int i) { return i; });
     b->id_groups[prec].clear();
     b->id_group_lds[prec].clear();
     for (int i = 0; i < R;) {
@@ -1210,11 +1225,11 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
           << key.n256 << ' ' << vs[best].block << ' ' << vs[best].npt << ' ' << vs[best].flags
           << ' ' << best_us << '\n';
   }
-  // Whole-batch check: buckets timed alone can mispredict the batch, whose
-  // launch groups run side by side; keep the tuned buckets only if the full
-  // batch (one solve per RVE, 4x the candidate steps) runs faster with them
-  // than with the plan they replace.
-  bool keep = true;
+  // Whole-batch check (also for buckets read from the cache): buckets timed
+  // alone can mispredict the batch, whose launch groups run side by side;
+  // keep the tuned buckets only if the full batch (one solve per RVE, 4x the
+  // candidate steps) runs faster with them than with the plan they replace.
+  // The per-bucket timings go to the cache either way.
   if (err == MF_OK && b->tuned != tuned0) {
     SolveSet S;
     S.prec = prec;
@@ -1244,8 +1259,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
     err = batch_ms(ms_old);
     b->tuned = tuned1;
     if (err == MF_OK) err = batch_ms(ms_new);
-    keep = err == MF_OK && ms_new < ms_old;
-    if (!keep) b->tuned = tuned0;
+    if (!(err == MF_OK && ms_new < ms_old)) b->tuned = tuned0;
   }
   b->p = p0;
   b->plan[prec] = plan0;
@@ -1257,7 +1271,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
   int e = make_plan(b, prec);
   if (e != MF_OK) return e;
   if ((e = build_identity_tables(b)) != MF_OK) return e;
-  if (keep && cache_path && !added.str().empty()) {
+  if (cache_path && !added.str().empty()) {
     const bool fresh = !std::ifstream(cache_path).good();
     std::ofstream f(cache_path, std::ios::app);
     if (!f) {

@@ -141,11 +141,18 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
     const double *P = pos + (kDbuf ? buf * 3 * Np : 0);
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int D = __builtin_amdgcn_readfirstlane(Dr[s]);
+    // rolling prefetch: entry j+1's list word and l0 load while entry j is
+    // evaluated (padding entries exist up to the slice width)
+    int32_t wv_n = D > 0 ? Ew[baser[s]] : 0;
+    double l0_n = D > 0 ? El0[baser[s]] : 1.0;
     for (int j = 0; j < D; ++j) {
+      const int32_t wv = wv_n;
+      const double l0 = l0_n;
+      if (j + 1 < D) {
+        wv_n = Ew[baser[s] + (j + 1) * kWave];
+        l0_n = El0[baser[s] + (j + 1) * kWave];
+      }
       if (j < degr[s]) {
-        const int idx = baser[s] + j * kWave;
-        const int32_t wv = Ew[idx];
-        const double l0 = El0[idx];
         const int w = wv & 0xFFFFFF;
         const double dx = P[w] - xr[s][0];
         const double dy = P[Np + w] - xr[s][1];

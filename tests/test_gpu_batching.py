@@ -63,10 +63,15 @@ def test_autotune_keeps_parity_and_uses_the_cache(tmp_path):
     err = rel_err(s32, so)
     say(f"  timed {nt32}, plan {plan32[:, :3].tolist()}, vs oracle {err:.2e}")
     assert nt32 > 0 and err <= 1e-4
+    # the second batch takes every bucket from the cache (nothing re-timed);
+    # the whole-batch check then keeps the tuned or the heuristic plan by
+    # timing, and the same plan gives the same bits
     again = mf.BatchedFiberRVEAnalysisExplicit(nets, p32)
     assert again.autotune(steps=100, cache_path=cache) == 0
-    assert np.array_equal(again.plan()[1], plan32)
-    assert np.array_equal(again.run(F, update_coords=False), s32)
+    s_again = again.run(F, update_coords=False)
+    assert rel_err(s_again, so) <= 1e-4
+    if np.array_equal(again.plan()[1], plan32):
+        assert np.array_equal(s_again, s32)
     with open(cache) as f:
         lines = [l for l in f if not l.startswith("#")]
     assert len(lines) >= 2
