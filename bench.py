@@ -198,6 +198,41 @@ def bench_cfg4(args):
            "value": round(13 * R / wall, 3), "unit": "RVE solves/s (incl. perturbed)",
            "tangents_per_s": round(R / wall, 3), "ms_per_batch": round(1e3 * wall, 3),
            "dtype": "f64"}
+    # the same batch with FP32c solves (FP64 stress and differencing): speed
+    # and error of sigma and C against the FP64 GPU results, at the config's
+    # h and at h = 1e-4 (FP32 rounding of the +-h solves against the 2h
+    # signal sets the error at small h)
+    g64_1e4 = None
+    fp32 = {}
+    for h in (1e-6, 1e-4):
+        p32 = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S,
+                                reaction_policy=mf.MF_REACT_PER_FIBER,
+                                tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=h,
+                                precision=mf.MF_PREC_FP32C, tangent_precision=mf.MF_PREC_FP32C)
+        g32 = mf.BatchedFiberRVEAnalysisExplicit(params=p32, generated=(gen, R), n_nodes=sizes)
+        s32, C32 = g32.run_with_tangent(F)  # from the reference state, like `first`
+        t = time.perf_counter()
+        g32.run_with_tangent(F)
+        w32 = time.perf_counter() - t
+        g32.close()
+        if h == 1e-6:
+            s_ref, C_ref = sigma, Cm
+        else:
+            p64 = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S,
+                                    reaction_policy=mf.MF_REACT_PER_FIBER,
+                                    tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=h)
+            g64_1e4 = mf.BatchedFiberRVEAnalysisExplicit(params=p64, generated=(gen, R),
+                                                         n_nodes=sizes)
+            s_ref, C_ref = g64_1e4.run_with_tangent(F)
+            g64_1e4.close()
+        eC = rel_err(C32.reshape(R, 36), C_ref.reshape(R, 36))
+        fp32[f"h={h:g}"] = {"solves_per_s": round(13 * R / w32, 3),
+                            "speedup_vs_fp64": round(wall / w32, 3),
+                            "sigma_err_vs_fp64_max": float(rel_err(s32, s_ref).max()),
+                            "tangent_err_vs_fp64_max": float(eC.max()),
+                            "tangent_err_vs_fp64_median": float(np.median(eC))}
+        log(f"[cfg4] fp32c solves, h={h:g}: {fp32[f'h={h:g}']}")
+    res["fp32c_solves"] = fp32
     if not args.no_cpu:
         n = 16
         nets = [mf.generate_network(seed=200001 + r, n_nodes=int(sizes[r]),
