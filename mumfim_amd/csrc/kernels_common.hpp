@@ -248,14 +248,16 @@ struct Lin64 {
 #define MF_FP32C_SPEC_LIST(X)
 
 // Fiber-streamed FP32c variants (large RVEs): (threads, fibers per thread,
-// free nodes per thread, reaction mode flag kLinearOnly|kMix2); the
-// instantiations whose step loop stays spill-free (7 fibers per thread with
-// the toe law would spill).
+// free nodes per thread, reaction mode flag kLinearOnly|kMix2).  The widest
+// ones spill a few registers in the step loop (7 toe-law or 8 linear fibers
+// per thread) and still beat the LDS-list kernel; 8 toe-law fibers per
+// thread did not.
 #define MF_FIBER_STREAM_LIST(X)                                              \
   X(512, 4, 2, kLinearOnly) X(512, 4, 2, kMix2)                              \
   X(1024, 4, 2, kLinearOnly) X(1024, 4, 2, kMix2)                            \
   X(1024, 6, 3, kLinearOnly) X(1024, 6, 3, kMix2)                            \
-  X(1024, 7, 3, kLinearOnly)
+  X(1024, 7, 3, kLinearOnly) X(1024, 7, 3, kMix2)                            \
+  X(1024, 8, 3, kLinearOnly)
 
 // Fiber-parallel FP32c variants: (threads, fibers per thread, free nodes per
 // thread, reaction mode flags kLinearOnly|kMix2, 0 = reaction table).
