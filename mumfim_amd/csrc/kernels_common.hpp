@@ -279,6 +279,8 @@ struct Lin64 {
   X(MF_PREC_FP64, 512, 2, 2) X(MF_PREC_FP64, 512, 4, 2)                      \
   X(MF_PREC_FP64, 512, 4, 0) X(MF_PREC_FP64, 1024, 2, 2)                     \
   X(MF_PREC_FP64, 1024, 2, 0) X(MF_PREC_FP64, 1024, 4, 0)                    \
+  X(MF_PREC_FP64, 512, 1, 3) X(MF_PREC_FP64, 512, 1, 2)                      \
+  X(MF_PREC_FP64, 1024, 1, 2)                                                \
   X(MF_PREC_FP32C, 256, 1, 3) X(MF_PREC_FP32C, 256, 1, 2)                    \
   X(MF_PREC_FP32C, 256, 2, 3) X(MF_PREC_FP32C, 256, 2, 2)                    \
   X(MF_PREC_FP32C, 256, 4, 3) X(MF_PREC_FP32C, 256, 4, 2)                    \
