@@ -271,7 +271,9 @@ bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE
   if (b->p.block_size && v.block != b->p.block_size) return false;
   const bool regs = (v.flags & kEllInRegs) != 0;
   if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) return false;
-  if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block)) return false;
+  if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block ||
+               p.N > kRegBufStride))
+    return false;
   if ((v.flags & kLinearOnly) && !lin) return false;
   if ((v.flags & kMix2) && !mix2_ok(p)) return false;
   if (((v.flags & kEnergy) != 0) != (prec == MF_PREC_FP64 && b->p.energy_diagnostics != 0))

@@ -205,7 +205,7 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
     // low parts at the end
     b += 16 * (Np + std::max<size_t>((size_t)ell_free_entries + kern::kWave, Np));
   } else if (v.flags & kEllInRegs) {
-    b += 16 * Np * 2;  // displacement buffers only
+    b += 16 * (kRegBufStride + Np);  // displacement buffers only, fixed stride
   } else {
     // displacement buffers + reference coordinates
     b += 16 * Np * ((v.flags & kDoubleBuf) ? 3 : 2);

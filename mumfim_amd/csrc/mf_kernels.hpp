@@ -61,6 +61,10 @@ enum : int32_t {
   kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
   kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
 };
+// Stride (float4 entries) of the two position buffers of the register
+// kernel: buffer 1 starts 32 KiB into LDS, a compile-time offset the LDS
+// loads take as an immediate; RVEs of up to this many nodes.
+constexpr int kRegBufStride = 2048;
 // Fibers per thread of a fiber-parallel variant (flags bits 8-11).
 constexpr int kFplShift = 8;
 inline int variant_fpl(int32_t flags) { return (flags >> kFplShift) & 15; }

@@ -12,6 +12,12 @@ namespace kern {
 // one nonlinear reaction, toe range within the expm1 polynomial's bound.
 enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
 
+// Pointers into LDS (address space 3): byte addresses within the
+// workgroup's LDS allocation.
+typedef __attribute__((address_space(3))) const unsigned char lds_uchar;
+typedef float v3f __attribute__((ext_vector_type(3), aligned(16)));
+typedef __attribute__((address_space(3))) const v3f lds_v3f;
+
 // ============================================================================
 // FP32 + compensated kernel
 // ============================================================================
@@ -499,13 +505,38 @@ constexpr int kLaneDeg = kLaneDegree;
 // EA/(2 l0^3) per fiber; RM_MIX2 keeps EA/(2 l0^3) for linear fibers and
 // -1/l0 for nonlinear ones, with the nonlinear law's constants uniform.
 
+// Waves per SIMD the register kernel is compiled for (minimum, maximum):
+// one node per thread at 256 or 512 threads in the linear and
+// one-linear-one-nonlinear modes runs four (two or four solves per CU, at
+// most 128 VGPRs); the reaction-table mode and several nodes per thread run
+// one solve per CU (768 threads: three waves, up to 168 VGPRs; 256 or 512
+// threads: two, up to 256); the other one-node shapes are left to the
+// compiler.
 template <int B, int NPT, int RM>
-__global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
+constexpr int reg_waves_min() {
+  return RM == RM_TABLE ? 2 : NPT == 1 ? (B <= 512 ? 4 : 1) : (B == 768 ? 3 : 2);
+}
+template <int B, int NPT, int RM>
+constexpr int reg_waves_max() {
+  return RM == RM_TABLE || NPT > 1 ? reg_waves_min<B, NPT, RM>() : 8;
+}
+
+template <int B, int NPT, int RM>
+__global__ __launch_bounds__(B)
+__attribute__((amdgpu_waves_per_eu(reg_waves_min<B, NPT, RM>(), reg_waves_max<B, NPT, RM>())))
+void solve32r(SolveArgs a) {
   constexpr bool LIN = RM == RM_LIN;
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
   const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  if (__builtin_amdgcn_groupstaticsize() != 0) {  // the LDS addressing below needs base 0
+    if (tid == 0) {
+      for (int i = 0; i < 6; ++i) a.sigma[6 * (int64_t)sid + i] = __builtin_nan("");
+      a.status[sid] = MF_STATUS_NAN;
+    }
+    return;
+  }
   const int rv = a.solve_rve[sid];
   const RveDev R = a.rves[rv];
   const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
@@ -513,8 +544,10 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   const int64_t nb = R.node_off;
 
   size_t off = 0;
-  float4 *pos = reinterpret_cast<float4 *>(smem);  // [2][Np] displacement hi
-  off += sizeof(float4) * 2 * Np;
+  // displacement hi, two buffers at a fixed stride: the buffer of a step is a
+  // compile-time constant, so it folds into the LDS loads' immediate offset
+  float4 *pos = reinterpret_cast<float4 *>(smem);  // [2][kRegBufStride]
+  off += sizeof(float4) * (kRegBufStride + Np);
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
   double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
@@ -557,7 +590,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     for (int k = 0; k < 3; ++k) u[k] = ramp ? up[k] : inc[k] + up[k];
     const float4 p = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
     pos[r] = p;
-    pos[Np + r] = p;
+    pos[kRegBufStride + r] = p;
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
@@ -656,16 +689,25 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   // internal force of a slot's node (both halves for a split node).  The
   // wave-uniform trip count selects a straight-line body, so the
   // neighbour-row loads issue ahead of the arithmetic.
-  auto node_force = [&](int sl, int buf, float (&f)[3], bool any_split) {
-    const unsigned char *P = reinterpret_cast<const unsigned char *>(pos + buf * Np);
+  auto node_force = [&](int sl, auto BUFC, float (&f)[3], bool any_split)
+                        __attribute__((always_inline)) {
+    constexpr int buf = decltype(BUFC)::value;
+    // LDS address of the buffer: dynamic LDS starts at 0 (the kernel has no
+    // static LDS, checked on entry), so neighbour offset + buffer constant is
+    // the whole address and the constant becomes the load's immediate offset
+    const lds_uchar *P = reinterpret_cast<const lds_uchar *>(
+        static_cast<uintptr_t>(buf * kRegBufStride * (int)sizeof(float4)));
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](auto KC) {
       constexpr int K = decltype(KC)::value;
       float4 q[K];
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        q[j] = *reinterpret_cast<const float4 *>(
+      {
+        const v3f v = *reinterpret_cast<const lds_v3f *>(
             P + (RM == RM_TABLE ? (offs[sl][j] & 0xFFFFFF) : offs[sl][j]));
+        q[j] = make_float4(v.x, v.y, v.z, 0.0f);
+      }
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         // both ends as their FP32-rounded displacements (the low parts only
@@ -732,7 +774,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   };
   // min fiber length^2 over all fibers (every dt refresh)
   auto min_length2 = [&](int buf) {
-    const float4 *P = pos + buf * Np;
+    const float4 *P = pos + buf * kRegBufStride;
     float lmin = 3.0e38f;
     const int32_t *fb = a.fib + 2 * R.fib_off;
     for (int e = tid; e < M; e += B) {
@@ -758,7 +800,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     double resid_local = 0.0;
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl)
-      if (own[sl]) node_force(sl, cur, fint[sl], true);
+      if (own[sl]) node_force(sl, std::integral_constant<int, 0>{}, fint[sl], true);
     double lmin2 = min_length2(cur);
     block_min_sum<B>(lmin2, resid_local, red);
     const float h = crit_half * (sqrtf((float)lmin2) * inv_c);
@@ -777,10 +819,12 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
   double residual = 10.0;
   int status = MF_STATUS_OK;
   const int freq = a.freq;
-  for (;;) {
+  // one step whose positions go to buffer NXT (a compile-time constant);
+  // returns true when the solve is done
+  auto one_step = [&](auto NXTC) __attribute__((always_inline)) -> bool {
+    constexpr int nxt = decltype(NXTC)::value;
     const float h2 = 2.0f * h_prev;
-    const int nxt = cur ^ 1;
-    float4 *Pn = pos + nxt * Np;
+    float4 *Pn = pos + nxt * kRegBufStride;
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl) {
       if (own[sl]) {
@@ -814,7 +858,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl) {
       if (own[sl]) {
-        node_force(sl, cur, fint[sl], sl == 0 && wave_split);
+        node_force(sl, NXTC, fint[sl], sl == 0 && wave_split);
         if (refresh && writer[sl])
           rsum += (double)(fint[sl][0] * fint[sl][0] + fint[sl][1] * fint[sl][1] +
                            fint[sl][2] * fint[sl][2]);
@@ -822,7 +866,7 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
     }
     float h = h_prev;
     if (refresh) {
-      double lm = min_length2(cur);
+      double lm = min_length2(nxt);
       block_min_sum<B>(lm, rsum, red);
       h = crit_half * (sqrtf((float)lm) * inv_c);
       residual = sqrt(rsum);
@@ -837,22 +881,25 @@ __global__ __launch_bounds__(B) void solve32r(SolveArgs a) {
       }
     h_prev = h;
     ++step;
-    if (ramp) {
-      if (step >= a.n_steps) break;
-    } else {
-      if (!(residual > a.tol)) break;
-      if (step >= a.max_steps) {
-        status = MF_STATUS_STEP_CAP;
-        break;
-      }
+    if (ramp) return step >= a.n_steps;
+    if (!(residual > a.tol)) return true;
+    if (step >= a.max_steps) {
+      status = MF_STATUS_STEP_CAP;
+      return true;
     }
+    return false;
+  };
+  // the positions start in buffer 0; steps alternate buffers 1, 0, 1, ...
+  for (;;) {
+    if (one_step(std::integral_constant<int, 1>{})) break;
+    if (one_step(std::integral_constant<int, 0>{})) break;
   }
   if (residual != residual) status = MF_STATUS_NAN;
 
   // finalize: low parts into the spare buffer, boundary gathers in FP32 with
   // exactly rounded reference vectors, stress in FP64
-  float4 *Ph = pos + cur * Np;
-  float4 *Pl = pos + (cur ^ 1) * Np;
+  float4 *Ph = pos + cur * kRegBufStride;
+  float4 *Pl = pos + (cur ^ 1) * kRegBufStride;
   __syncthreads();
 #pragma unroll
   for (int sl = 0; sl < NPT; ++sl)
