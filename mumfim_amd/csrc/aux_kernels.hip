@@ -166,10 +166,6 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kMix2},
     {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},
-    // LDS-list kernels specialised to one reaction mode (large RVEs)
-#define MF_SPEC_ENTRY(B, N, F, MFL) {MF_PREC_FP32C, B, N, (F) | (MFL)},
-    MF_FP32C_SPEC_LIST(MF_SPEC_ENTRY)
-#undef MF_SPEC_ENTRY
     // fiber-parallel: (threads, fibers per thread, free nodes per thread)
 #define MF_FPAR_ENTRY(B, FPL, NPL, FL) {MF_PREC_FP32C, B, NPL, kFiberPar | FL | ((FPL) << kFplShift)},
     MF_FIBER_PAR_LIST(MF_FPAR_ENTRY)

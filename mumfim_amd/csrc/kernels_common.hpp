@@ -243,10 +243,6 @@ struct Lin64 {
 }  // namespace kern
 }  // namespace mf
 
-// Mode-specialised FP32c LDS-list variants for large RVEs: (threads, node
-// slots per thread, flags kEllInLds|kDoubleBuf, reaction mode flag).
-#define MF_FP32C_SPEC_LIST(X)
-
 // Fiber-streamed FP32c variants (large RVEs): (threads, fibers per thread,
 // free nodes per thread, reaction mode flag kLinearOnly|kMix2).  The widest
 // ones spill a few registers in the step loop (7 toe-law or 8 linear fibers

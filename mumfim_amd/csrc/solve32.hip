@@ -21,23 +21,17 @@ typedef __attribute__((address_space(3))) const v3f lds_v3f;
 // ============================================================================
 // FP32 + compensated kernel
 // ============================================================================
-__device__ inline void two_sum(float a, float b, float &s, float &e) {
-  s = a + b;
-  float bp = s - a;
-  e = (a - (s - bp)) + (b - bp);
-}
-
 // State per free node: displacement u = (hi, lo) in FP32 advanced with
-// TwoSum, velocity and acceleration in FP32.  LDS holds the displacement hi
+// Fast2Sum, velocity in FP32.  LDS holds the displacement hi
 // parts (double-buffered) and the reference coordinates X (FP32).  A fiber
 // seen from node v towards w has reference vector d0 = X_w - X_v,
 // displacement difference delta = u_w - u_v, current vector d = d0 + delta
 // and l^2 - l0^2 = (d0 + d) . delta, which keeps full relative precision at
 // small strain (the direct form d.d - l0^2 loses eps/strain).
-// RM < 0: the reaction mode is read per RVE and each incidence entry is
-// evaluated in a loop; RM = RM_LIN / RM_MIX2: mode-specialised straight-line
-// gathers (variants for large RVEs).
-template <int B, int NPT, int FL, int RM = -1>
+// The reaction mode is read per RVE (linear-only, one linear and one
+// nonlinear reaction, or the packed table) and is uniform over the
+// workgroup.
+template <int B, int NPT, int FL>
 __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   constexpr bool kEll = (FL & kEllInLds) != 0;
   constexpr bool kDbuf = (FL & kDoubleBuf) != 0;
@@ -156,77 +150,15 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   int cur = 0;
   // internal force of a slot's node.  ELL padding names the node itself
   // with 1/l0 = 0 (zero force), so every entry of the slice width is
-  // evaluated unmasked.  In the mode-specialised variants the wave-uniform
-  // width selects straight-line batches of 4 entries whose list and
-  // position loads issue together.
+  // evaluated unmasked; entry j+1's list word and 1/l0 load while entry j is
+  // evaluated (rolling prefetch: the lists stream from HBM for large RVEs).
   auto node_force = [&](int s, int buf, float (&f)[3]) {
     const float4 *P = pos + (kDbuf ? buf * Np : 0);
     const int4 Xv = Xs[s * B + tid];  // own reference coordinate
     const int base = sbase[s] + lane;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    auto entry = [&](auto MC, int32_t wv, float il, const float4 &q, const int4 &xq) {
-      constexpr int MODE = decltype(MC)::value;
-      // reference vector: exact integer difference, one rounding to FP32
-      const float ex = (float)(xq.x - Xv.x) * xq_inv;
-      const float ey = (float)(xq.y - Xv.y) * xq_inv;
-      const float ez = (float)(xq.z - Xv.z) * xq_inv;
-      const float gx = (q.x - uh[s][0]) - ul[s][0];
-      const float gy = (q.y - uh[s][1]) - ul[s][1];
-      const float gz = (q.z - uh[s][2]) - ul[s][2];
-      const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      // l^2 - l0^2 = (d0 + d) . delta
-      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
-      const float e2 = sl * (il * il);
-      const unsigned rid = (uint32_t)wv >> 24;
-      float t;
-      if constexpr (MODE == RM_LIN)
-        t = lin_EA_half * il * e2;
-      else if constexpr (MODE == RM_MIX2)
-        t = react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il);
-      else
-        t = lin_only ? lin_EA_half * il * e2
-            : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
-                     : react32_tab(rtab, rid, e2, il);
-      fx -= dx * t;
-      fy -= dy * t;
-      fz -= dz * t;
-    };
-    auto batch = [&](auto MC, auto J0C, auto KC) {
-      constexpr int J0 = decltype(J0C)::value, K = decltype(KC)::value;
-      int32_t wv[K];
-      float il[K];
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const int idx = base + (J0 + j) * kWave;
-        wv[j] = Ew[idx];
-        il[j] = Eil[idx];
-      }
-      float4 q[K];
-      int4 xq[K];
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const int w = wv[j] & 0xFFFFFF;
-        q[j] = P[w];
-        xq[j] = Xs[w];
-      }
-#pragma unroll
-      for (int j = 0; j < K; ++j) entry(MC, wv[j], il[j], q[j], xq[j]);
-    };
-    auto body = [&](auto MC, auto KC) {
-      constexpr int K = decltype(KC)::value;
-      using I = std::integral_constant<int, 0>;
-      batch(MC, I{}, std::integral_constant<int, (K < 4 ? K : 4)>{});
-      if constexpr (K > 4)
-        batch(MC, std::integral_constant<int, 4>{}, std::integral_constant<int, (K < 8 ? K - 4 : 4)>{});
-      if constexpr (K > 8)
-        batch(MC, std::integral_constant<int, 8>{}, std::integral_constant<int, K - 8>{});
-    };
-    using MC = std::integral_constant<int, RM>;
-    // rolling prefetch: entry j+1's list word and 1/l0 load while entry j
-    // is evaluated
-    auto loop = [&]() {
-      const int D = Dr[s];
-      if (D <= 0) return;
+    const int D = Dr[s];
+    if (D > 0) {
       int32_t wv_n = Ew[base];
       float il_n = Eil[base];
       for (int j = 0; j < D; ++j) {
@@ -237,24 +169,26 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
           il_n = Eil[base + (j + 1) * kWave];
         }
         const int w = wv & 0xFFFFFF;
-        entry(MC{}, wv, il, P[w], Xs[w]);
-      }
-    };
-    if constexpr (RM < 0) {
-      loop();
-    } else {
-      switch (Dr[s]) {
-#define MF_DEG_CASE(k) \
-  case k:              \
-    body(MC{}, std::integral_constant<int, k>{}); \
-    break;
-        MF_DEG_CASE(1) MF_DEG_CASE(2) MF_DEG_CASE(3) MF_DEG_CASE(4)
-        MF_DEG_CASE(5) MF_DEG_CASE(6) MF_DEG_CASE(7) MF_DEG_CASE(8)
-        MF_DEG_CASE(9) MF_DEG_CASE(10) MF_DEG_CASE(11) MF_DEG_CASE(12)
-#undef MF_DEG_CASE
-        default:
-          loop();
-          break;
+        const float4 q = P[w];
+        const int4 xq = Xs[w];
+        // reference vector: exact integer difference, one rounding to FP32
+        const float ex = (float)(xq.x - Xv.x) * xq_inv;
+        const float ey = (float)(xq.y - Xv.y) * xq_inv;
+        const float ez = (float)(xq.z - Xv.z) * xq_inv;
+        const float gx = (q.x - uh[s][0]) - ul[s][0];
+        const float gy = (q.y - uh[s][1]) - ul[s][1];
+        const float gz = (q.z - uh[s][2]) - ul[s][2];
+        const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+        // l^2 - l0^2 = (d0 + d) . delta
+        const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+        const float e2 = sl * (il * il);
+        const unsigned rid = (uint32_t)wv >> 24;
+        const float t = lin_only ? lin_EA_half * il * e2
+                        : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
+                                 : react32_tab(rtab, rid, e2, il);
+        fx -= dx * t;
+        fy -= dy * t;
+        fz -= dz * t;
       }
     }
     f[0] = fx;
@@ -1891,18 +1825,6 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
-#define MF_SPEC_LAUNCH(B, N, F, MFL)                                          \
-  if (v.block == B && v.npt == N && v.flags == ((F) | (MFL))) {               \
-    auto k = kern::solve32c<B, N, F, (MFL) == kLinearOnly ? kern::RM_LIN : kern::RM_MIX2>; \
-    if (lds > 64 * 1024)                                                      \
-      (void)hipFuncSetAttribute((const void *)k,                              \
-                                hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                                (int)kMaxLds);                                \
-    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);                \
-    return hipGetLastError();                                                 \
-  }
-  MF_FP32C_SPEC_LIST(MF_SPEC_LAUNCH)
-#undef MF_SPEC_LAUNCH
 #define MF_REG_LAUNCH(B, N, RMODE, FLAG)                                       \
   if (v.npt == N && v.block == B && (v.flags & kEllInRegs) &&                  \
       (v.flags & (kLinearOnly | kMix2)) == FLAG) {                             \
