@@ -255,6 +255,13 @@ def test_edge_cases_all_boundary_and_no_boundary():
         s_g = mf.BatchedFiberRVEAnalysisExplicit(nets, p).run(np.stack([F, F]))
         assert rel_err(s_g[:1], s_o[:1]) <= FP64_TOL
         assert np.all(s_g[1] == 0) and np.all(s_o[1] == 0)
+        # the FP32c families with no free node and with no boundary node
+        for fk in (mf.MF_FP32_KERNEL_NODE, mf.MF_FP32_KERNEL_FIBER):
+            p32 = mf.default_params(mode=mode, n_steps=300, precision=mf.MF_PREC_FP32C,
+                                    fp32_kernel=fk)
+            s32 = mf.BatchedFiberRVEAnalysisExplicit(nets, p32).run(np.stack([F, F]))
+            assert rel_err(s32[:1], s_o[:1]) <= FP32C_TOL
+            assert np.all(s32[1] == 0)
 
 
 def test_status_nan_and_step_cap():
