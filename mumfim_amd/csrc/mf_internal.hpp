@@ -65,8 +65,9 @@ constexpr int kSlice = 64;  // nodes per ELL slice = one wavefront
 // degree (up to twice this) take two lanes.
 constexpr int kLaneDegree = 6;
 
-// One RVE packed in kernel order: free nodes sorted by degree (descending,
-// stable), then boundary nodes in boundary-list order.
+// One RVE packed in kernel order: free nodes sorted by degree (descending)
+// and within a degree by the Morton code of X, then boundary nodes in
+// boundary-list order.
 struct PackedRVE {
   int N = 0, Nf = 0, Nb = 0, M = 0;
   std::vector<int32_t> rank_of_input;  // input node -> kernel rank

@@ -322,14 +322,42 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
     deg_in[net.fib[2 * e]]++;
     deg_in[net.fib[2 * e + 1]]++;
   }
-  // kernel order: free nodes (input order, stable-sorted by degree desc),
+  // kernel order: free nodes (sorted by degree desc, then Morton order),
   // then boundary nodes in boundary-list order (BatchedReorderMesh.h:54-73)
   std::vector<int> free_nodes;
   free_nodes.reserve(N - Nb);
   for (int i = 0; i < N; ++i)
     if (!is_bnd[i]) free_nodes.push_back(i);
-  std::stable_sort(free_nodes.begin(), free_nodes.end(),
-                   [&](int a, int b) { return deg_in[a] > deg_in[b]; });
+  // within a degree, Morton order of X: lanes of a wave then gather
+  // neighbours that sit close together in LDS
+  std::vector<uint32_t> morton(N, 0);
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], net.X[3 * i + d]);
+        hi[d] = std::max(hi[d], net.X[3 * i + d]);
+      }
+    auto spread = [](uint32_t v) {  // 10 bits -> every third of 30 bits
+      v &= 0x3FF;
+      v = (v | (v << 16)) & 0x030000FF;
+      v = (v | (v << 8)) & 0x0300F00F;
+      v = (v | (v << 4)) & 0x030C30C3;
+      v = (v | (v << 2)) & 0x09249249;
+      return v;
+    };
+    for (int i = 0; i < N; ++i) {
+      uint32_t c[3];
+      for (int d = 0; d < 3; ++d) {
+        const double w = hi[d] > lo[d] ? (net.X[3 * i + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+        c[d] = (uint32_t)std::min(1023.0, std::max(0.0, std::floor(w * 1024.0)));
+      }
+      morton[i] = spread(c[0]) | (spread(c[1]) << 1) | (spread(c[2]) << 2);
+    }
+  }
+  std::stable_sort(free_nodes.begin(), free_nodes.end(), [&](int a, int b) {
+    return deg_in[a] != deg_in[b] ? deg_in[a] > deg_in[b] : morton[a] < morton[b];
+  });
   p.input_of_rank.resize(N);
   p.rank_of_input.resize(N);
   for (int r = 0; r < (int)free_nodes.size(); ++r) p.input_of_rank[r] = free_nodes[r];
