@@ -332,8 +332,12 @@ int make_plan(mf_batch *b, int prec) {
       // waves hide the FP64 division and square-root chains: 1000-2000 node
       // RVEs run 1.4x faster on 1024 threads x 2 slots than on 512 x 4).
       const long fpl = variant_fpl(v.flags);
+      // 256 threads x 2 linear nodes per thread (three solves per CU) beats
+      // one node per thread at 512+ threads: 9% on 500-node linear RVEs
+      const bool reg256x2 = regs && v.block == 256 && v.npt == 2;
       long key = fpar   ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
                  : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
+                 : reg256x2 ? 10000000L + 5000
                  : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
                         : 1000000000L + (long)v.npt * 10000000 + (long)v.block * 1000 - v.flags;
       if (best < 0 || key < best_key) {

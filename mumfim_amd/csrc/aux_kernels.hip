@@ -163,6 +163,7 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf},
     // several free nodes per thread
     {MF_PREC_FP32C, 768, 2, kEllInRegs | kDoubleBuf | kLinearOnly},
+    {MF_PREC_FP32C, 256, 2, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kLinearOnly},
     {MF_PREC_FP32C, 512, 3, kEllInRegs | kDoubleBuf | kMix2},
     {MF_PREC_FP32C, 512, 4, kEllInRegs | kDoubleBuf | kLinearOnly},

@@ -443,12 +443,13 @@ constexpr int kLaneDeg = kLaneDegree;
 // one node per thread at 256 or 512 threads in the linear and
 // one-linear-one-nonlinear modes runs four (two or four solves per CU, at
 // most 128 VGPRs); the reaction-table mode and several nodes per thread run
-// one solve per CU (768 threads: three waves, up to 168 VGPRs; 256 or 512
-// threads: two, up to 256); the other one-node shapes are left to the
-// compiler.
+// one solve per CU (768 threads: three waves, up to 168 VGPRs; 512
+// threads: two, up to 256), except 256 threads with two nodes per thread,
+// which runs three solves per CU (three waves, up to 168 VGPRs); the other
+// one-node shapes are left to the compiler.
 template <int B, int NPT, int RM>
 constexpr int reg_waves_min() {
-  return RM == RM_TABLE ? 2 : NPT == 1 ? (B <= 512 ? 4 : 1) : (B == 768 ? 3 : 2);
+  return RM == RM_TABLE ? 2 : NPT == 1 ? (B <= 512 ? 4 : 1) : (B == 768 || B == 256 ? 3 : 2);
 }
 template <int B, int NPT, int RM>
 constexpr int reg_waves_max() {
@@ -1850,6 +1851,7 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
   MF_REG_LAUNCH2(1024, 1)
   // several nodes per thread
   MF_REG_LAUNCH(768, 2, kern::RM_LIN, kLinearOnly)
+  MF_REG_LAUNCH(256, 2, kern::RM_LIN, kLinearOnly)
   MF_REG_LAUNCH(512, 3, kern::RM_LIN, kLinearOnly)
   MF_REG_LAUNCH(512, 3, kern::RM_MIX2, kMix2)
   MF_REG_LAUNCH(512, 4, kern::RM_LIN, kLinearOnly)
