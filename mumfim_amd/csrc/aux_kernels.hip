@@ -191,7 +191,8 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
     // 12-byte displacement and force records (+ zero record), scratch,
     // boundary state, packed table
     const size_t nff = (std::max<size_t>((size_t)M + 1, (size_t)N) + 1) & ~size_t(1);
-    b = 12 * (Np + nff) + sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp;
+    b = (12 * (Np + nff) + 15) & ~size_t(15);
+    b += sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp;
     b = (b + 15) & ~size_t(15);
     return b + sizeof(float) * 4 * 3 * kMaxReactions;
   }

@@ -72,6 +72,26 @@ __device__ inline double ramp_alpha(int k, int S, int amplitude) {
   return xi;
 }
 
+// Ramp factor of the FP32 kernels' step loop, whose boundary positions are
+// rounded to FP32: xi = k * (1/S) is within an ulp of k / S and saves an FP64
+// division per step.  The final stress uses ramp_alpha.
+__device__ inline double ramp_alpha_step(int k, double inv_S, int amplitude) {
+  const double xi = (double)k * inv_S;
+  if (amplitude == MF_AMP_SMOOTHSTEP) return xi * xi * xi * (10 - 15 * xi + 6 * xi * xi);
+  return xi;
+}
+
+// Boundary displacement u_b = u0 + alpha g of the FP32 kernels from a
+// boundary node's 48-byte LDS record (u0x u0y u0z gx gy gz, 16-byte aligned),
+// read as three 16-byte pairs.
+__device__ inline void bnd_disp(const double *rec, double alpha, double (&ub)[3]) {
+  const double2 *p = reinterpret_cast<const double2 *>(rec);
+  const double2 q0 = p[0], q1 = p[1], q2 = p[2];
+  ub[0] = fma(alpha, q1.y, q0.x);
+  ub[1] = fma(alpha, q2.x, q0.y);
+  ub[2] = fma(alpha, q2.y, q1.x);
+}
+
 // min of a (slot 0) and sum of b (slot 1) in one pass.
 template <int B>
 __device__ inline void block_min_sum(double &a, double &b, double *red) {

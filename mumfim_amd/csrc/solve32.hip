@@ -52,7 +52,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   off += sizeof(int4) * Np;
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  double *bst = reinterpret_cast<double *>(smem + off);  // [Nbp][6] u_b0, g
   off += sizeof(double) * 6 * Nbp;
   float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
   off += sizeof(float4) * kReactTabStride * kMaxReactions;
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
+  const double inv_S = 1.0 / a.n_steps;
 
   float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
   // slice width and base are wave-uniform (scalar registers)
@@ -115,8 +116,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const int b = r - Nf;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[k * Nbp + b] = u[k];
-        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+        bst[6 * b + k] = u[k];
+        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
       }
     }
   };
@@ -265,12 +266,12 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       }
     }
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
-      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
-      for (int b = tid; b < Nb; b += B)
-        Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
-                                 (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
-                                 (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
-                                 0.0f);
+      const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+        double ub[3];
+        bnd_disp(bst + 6 * b, alpha, ub);
+        Pn[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+      }
     }
     __syncthreads();
     cur = nxt;
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
-    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    bnd_disp(bst + 6 * b, alpha_end, ub);
     const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
     Ph[Nf + b] = hb;
     if (kDbuf)
@@ -345,10 +346,8 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     const int r = Nf + b;
     const int64_t gr = nb + r;
     double ub[3], xb[3];
-    for (int d = 0; d < 3; ++d) {
-      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-      xb[d] = a.X[3 * gr + d] + ub[d];
-    }
+    bnd_disp(bst + 6 * b, alpha_end, ub);
+    for (int d = 0; d < 3; ++d) xb[d] = a.X[3 * gr + d] + ub[d];
     const float4 hb = Ph[r];
     const int4 Xb = Xs[r];
     const float4 lb = kDbuf ? Pl[r] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -485,7 +484,7 @@ void solve32r(SolveArgs a) {
   off += sizeof(float4) * (kRegBufStride + Np);
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  double *bst = reinterpret_cast<double *>(smem + off);  // [Nbp][6] u_b0, g
   off += sizeof(double) * 6 * Nbp;
   float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
 
@@ -506,6 +505,7 @@ void solve32r(SolveArgs a) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
+  const double inv_S = 1.0 / a.n_steps;
 
   auto init_node = [&](int r, double (&u)[3]) {
     const int64_t g = nb + r;
@@ -530,8 +530,8 @@ void solve32r(SolveArgs a) {
       const int b = r - Nf;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[k * Nbp + b] = u[k];
-        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+        bst[6 * b + k] = u[k];
+        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
       }
     }
   };
@@ -779,12 +779,12 @@ void solve32r(SolveArgs a) {
       }
     }
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
-      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
-      for (int b = tid; b < Nb; b += B)
-        Pn[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
-                                 (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
-                                 (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
-                                 0.0f);
+      const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+        double ub[3];
+        bnd_disp(bst + 6 * b, alpha, ub);
+        Pn[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+      }
     }
     __syncthreads();
     cur = nxt;
@@ -842,7 +842,7 @@ void solve32r(SolveArgs a) {
   const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
-    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    bnd_disp(bst + 6 * b, alpha_end, ub);
     const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
     Ph[Nf + b] = hb;
     Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
@@ -855,10 +855,8 @@ void solve32r(SolveArgs a) {
     const int rb = Nf + b;
     const int64_t gr = nb + rb;
     double ub[3], xb[3];
-    for (int d = 0; d < 3; ++d) {
-      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-      xb[d] = X[3 * rb + d] + ub[d];
-    }
+    bnd_disp(bst + 6 * b, alpha_end, ub);
+    for (int d = 0; d < 3; ++d) xb[d] = X[3 * rb + d] + ub[d];
     const float4 hb = Ph[rb], lb = Pl[rb];
     const int k = rb / kWave;
     const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
@@ -962,7 +960,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
   off += sizeof(float4) * nslot;
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  double *bst = reinterpret_cast<double *>(smem + off);  // [Nbp][6] u_b0, g
   off += sizeof(double) * 6 * Nbp;
   float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
 
@@ -982,6 +980,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
+  const double inv_S = 1.0 / a.n_steps;
 
   // initial displacement: relax adds the affine increment (F - I) x to u,
   // ramp keeps u and sets the boundary increment aside
@@ -1009,8 +1008,8 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       const int b = r - Nf;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[k * Nbp + b] = u[k];
-        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+        bst[6 * b + k] = u[k];
+        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
       }
     }
   }
@@ -1245,12 +1244,12 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       }
     }
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
-      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
-      for (int b = tid; b < Nb; b += B)
-        pos[Nf + b] = make_float4((float)(bst[b] + alpha * bst[3 * Nbp + b]),
-                                  (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
-                                  (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b]),
-                                  0.0f);
+      const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+        double ub[3];
+        bnd_disp(bst + 6 * b, alpha, ub);
+        pos[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+      }
     }
     __syncthreads();  // positions published; every slot read of the last step done
     const bool refresh = (step % freq) == 0;
@@ -1308,7 +1307,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
   const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
-    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    bnd_disp(bst + 6 * b, alpha_end, ub);
     const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
     Ph[Nf + b] = hb;
     Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
@@ -1321,10 +1320,8 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
     const int rb = Nf + b;
     const int64_t gr = nb + rb;
     double ub[3], xb[3];
-    for (int d = 0; d < 3; ++d) {
-      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-      xb[d] = X[3 * rb + d] + ub[d];
-    }
+    bnd_disp(bst + 6 * b, alpha_end, ub);
+    for (int d = 0; d < 3; ++d) xb[d] = X[3 * rb + d] + ub[d];
     const float4 hb = Ph[rb], lb = Pl[rb];
     const int k = rb / kWave;
     const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
@@ -1431,9 +1428,10 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   off += sizeof(F3) * Np;
   F3 *ffr = reinterpret_cast<F3 *>(smem + off);  // [nff] fiber forces t d
   off += sizeof(F3) * nff;
+  off = (off + 15) & ~size_t(15);  // 16-byte boundary records below
   double *red = reinterpret_cast<double *>(smem + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);  // u_b0, g per boundary node
+  double *bst = reinterpret_cast<double *>(smem + off);  // [Nbp][6] u_b0, g
   off += sizeof(double) * 6 * Nbp;
   off = (off + 15) & ~size_t(15);
   float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed table (finalize)
@@ -1455,6 +1453,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
   const bool ramp = a.mode == MF_MODE_RAMP;
+  const double inv_S = 1.0 / a.n_steps;
 
   auto init_u = [&](int r, double (&u)[3], double (&inc)[3]) {
     const int64_t g = nb + r;
@@ -1480,8 +1479,8 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       const int b = r - Nf;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[k * Nbp + b] = u[k];
-        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+        bst[6 * b + k] = u[k];
+        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
       }
     }
   }
@@ -1661,11 +1660,12 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       }
     }
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
-      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
-      for (int b = tid; b < Nb; b += B)
-        pos[Nf + b] = F3{(float)(bst[b] + alpha * bst[3 * Nbp + b]),
-                         (float)(bst[Nbp + b] + alpha * bst[4 * Nbp + b]),
-                         (float)(bst[2 * Nbp + b] + alpha * bst[5 * Nbp + b])};
+      const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+        double ub[3];
+        bnd_disp(bst + 6 * b, alpha, ub);
+        pos[Nf + b] = F3{(float)ub[0], (float)ub[1], (float)ub[2]};
+      }
     }
     __syncthreads();  // positions published; every force read of the last step done
     const bool refresh = (step % freq) == 0;
@@ -1723,7 +1723,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
   for (int b = tid; b < Nb; b += B) {
     double ub[3];
-    for (int d = 0; d < 3; ++d) ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    bnd_disp(bst + 6 * b, alpha_end, ub);
     const F3 hb{(float)ub[0], (float)ub[1], (float)ub[2]};
     Ph[Nf + b] = hb;
     Pl[Nf + b] = F3{(float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
@@ -1736,10 +1736,8 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     const int rb = Nf + b;
     const int64_t gr = nb + rb;
     double ub[3], xb[3];
-    for (int d = 0; d < 3; ++d) {
-      ub[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
-      xb[d] = X[3 * rb + d] + ub[d];
-    }
+    bnd_disp(bst + 6 * b, alpha_end, ub);
+    for (int d = 0; d < 3; ++d) xb[d] = X[3 * rb + d] + ub[d];
     const F3 hb = Ph[rb], lb = Pl[rb];
     const int k = rb / kWave;
     const int base = a.slice_base[R.slice_off + k] + (rb & (kWave - 1));
