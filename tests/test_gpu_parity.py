@@ -91,6 +91,22 @@ def test_network27_tangent_fp64_matches_oracle(golden, net27):
     xo, uo, _ = o.get_state()
     xg, ug, _ = g.get_state()
     assert np.abs(xg - xo).max() <= 1e-14 and np.array_equal(ug, uo)
+    # central scheme: 12 chained trials, the -h trial of a column starting
+    # where its +h trial ended (the oracle's scheme, pinned by the chain
+    # closed form)
+    pc = mf.default_params(tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-5)
+    oc = Oracle([net27], pc)
+    oc.run(F)
+    Cc_o = oc.computeMaterialStiffness()
+    gc = mf.BatchedFiberRVEAnalysisExplicit([net27], pc)
+    gc.run(F)
+    Cc_g, stc = gc.computeMaterialStiffness(return_status=True)
+    assert (stc == 0).all()
+    errc = rel_err(Cc_g, Cc_o)
+    say(f"  central C rel err {errc:.3e}")
+    assert errc <= 1e-8
+    # forward and central agree to the scheme's truncation and trial order
+    assert rel_err(Cc_g, C_g) < 5e-3
 
 
 def test_relax_multi_rve_broadcast_trial(net27):

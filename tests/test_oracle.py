@@ -174,6 +174,72 @@ def test_chain_closed_form():
     assert np.abs(s[1:]).max() < 1e-12 * abs(s[0])
 
 
+def _perturbation(i, h):
+    """Perturbation of Voigt direction i (BatchedFiberRVEAnalysisExplicit.h:
+    338-390): a normal stretch D1 = (1 - 2h)^(-1/2), or a symmetric shear
+    with principal stretches sqrt(l2), sqrt(l3), l2 = (1+h)/(1-h^2),
+    l3 = (1-h)/(1-h^2)."""
+    Fp = np.eye(3)
+    if i < 3:
+        Fp[i, i] = math.sqrt(1.0 / (1 - 2 * h))
+        return Fp
+    l2, l3 = (1 + h) / (1 - h * h), (1 - h) / (1 - h * h)
+    j, k = [(1, 2), (0, 2), (0, 1)][i - 3]
+    Fp[j, j] = Fp[k, k] = 0.5 * (math.sqrt(l2) + math.sqrt(l3))
+    Fp[j, k] = Fp[k, j] = 0.5 * (math.sqrt(l2) - math.sqrt(l3))
+    return Fp
+
+
+@pytest.mark.parametrize("scheme", [mf.MF_TANGENT_FORWARD, mf.MF_TANGENT_CENTRAL])
+def test_chain_tangent_closed_form(scheme):
+    """Relax-mode tangent of the straight chain from the closed form, forward
+    and central (the central scheme has no reference counterpart).  The
+    relaxed chain is straight between its end nodes a and b, every fiber at
+    the same stretch, so sigma = EA (|c|^2 - 1)/2 c c^T / V with c = x_b - x_a
+    (reference length 1) and V = det F of the committed state (trial runs
+    keep the volume).  Trials follow the reference sequence: a trial restores
+    the displacement but keeps the coordinates it ended with
+    (BatchedFiberRVEAnalysisExplicit.h:264-313), and the next trial applies
+    its increment to those, so the end nodes move to x_c + (F_q - I) x_prev."""
+    xs = np.array([-0.5, -0.31, -0.02, 0.2, 0.37, 0.5])
+    X = np.stack([xs, np.zeros(6), np.zeros(6)], 1)
+    fb = np.array([[i, i + 1] for i in range(5)], np.int32)
+    r = mf.Reaction(0, 0.01, 1e6, 1000.0)
+    net = mf.Network(X, fb, np.array([0, 5], np.int32), [r])
+    EA = r.E * math.pi * r.radius ** 2
+    F = np.array([[1.04, 0.02, 0.0], [0.01, 0.99, 0.0], [0.0, 0.0, 1.01]])
+    V = np.linalg.det(F)
+
+    def sigma(c):
+        return np.array(voigt(EA * (c @ c - 1) / 2 * np.outer(c, c) / V))
+
+    h = 1e-4
+    central = scheme == mf.MF_TANGENT_CENTRAL
+    o = Oracle([net], mf.default_params(tolerance=1e-12, tangent_scheme=scheme, fd_step=h))
+    s = o.run(F)[0]
+    xa, xb = F @ X[0], F @ X[5]
+    s0 = sigma(xb - xa)
+    assert np.abs(s - s0).max() <= 1e-12 * np.abs(s0).max()
+    C = o.computeMaterialStiffness()[0]
+    Cc, Cu = np.zeros((6, 6)), np.zeros((6, 6))
+    pa, pb = xa, xb
+    for i in range(6):
+        chained, fresh = [], []
+        for sg in ((1, -1) if central else (1,)):
+            Fq = _perturbation(i, sg * h)
+            pa, pb = xa + (Fq - np.eye(3)) @ pa, xb + (Fq - np.eye(3)) @ pb
+            chained.append(sigma(pb - pa))
+            fresh.append(sigma(Fq @ (xb - xa)))
+        for Cx, t in ((Cc, chained), (Cu, fresh)):
+            Cx[:, i] = (t[0] - t[1]) / (2 * h) if central else (t[0] - s0) / h
+    scale = np.abs(Cc).max()
+    assert np.abs(C - Cc).max() <= 1e-9 * scale
+    if central:
+        # the chaining is visible at O(h): the -h trial starts where the +h
+        # trial ended
+        assert np.abs(C - Cu).max() >= 0.2 * h * scale
+
+
 def test_relax_converges_from_perturbed_state():
     """Relax mode drives a chain displaced off equilibrium back to it."""
     xs = np.array([-0.5, -0.25, 0.0, 0.25, 0.5])
