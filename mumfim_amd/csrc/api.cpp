@@ -208,6 +208,8 @@ struct mf_batch {
   DevBuf<double> id_F;
   std::vector<std::pair<int, int>> id_groups[2];  // (variant, count) in order
   std::vector<size_t> id_group_lds[2];              // LDS bytes per group
+  std::vector<int64_t> id_group_gs[2];              // global state per solve
+  DevBuf<double> gstate;  // global-state variants' per-solve scratch
   DevBuf<double> lastF_dev;
   bool lastF_on_device = false;
   // energies of the last run() (FP64 energy diagnostics), 4 per RVE
@@ -267,6 +269,10 @@ bool mix2_ok(const PackedRVE &p) {
 // energy diagnostics) under the batch parameters.
 bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE &p,
                   bool lin) {
+  if (v.flags & kGlobalState)  // FP64, any size; also the FP32c fallback
+    return !(prec == MF_PREC_FP64 && b->p.energy_diagnostics) &&
+           (!b->p.block_size || v.block == b->p.block_size) &&
+           lds_bytes(v, p.N, p.Nb, p.M, p.ell_free_entries) <= kMaxLds;
   if (v.prec != prec) return false;
   if (b->p.block_size && v.block != b->p.block_size) return false;
   const bool regs = (v.flags & kEllInRegs) != 0;
@@ -335,7 +341,9 @@ int make_plan(mf_batch *b, int prec) {
       // 256 threads x 2 linear nodes per thread (three solves per CU) beats
       // one node per thread at 512+ threads: 9% on 500-node linear RVEs
       const bool reg256x2 = regs && v.block == 256 && v.npt == 2;
-      long key = fpar   ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
+      // global-state FP64 last: only where nothing else fits
+      long key = (v.flags & kGlobalState) ? 2000000000L
+                 : fpar ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
                  : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
                  : reg256x2 ? 10000000L + 5000
                  : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
@@ -449,7 +457,12 @@ int build_identity_tables(mf_batch *b);
 void fill_args(mf_batch *b, SolveArgs &a);
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
-                  const std::vector<size_t> &lds);
+                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride);
+// doubles of global state per solve of a group of variant v whose largest
+// RVE has N nodes (global-state variants; 0 otherwise)
+int64_t group_gstride(const Variant &v, int N) {
+  return (v.flags & kGlobalState) ? 12 * (int64_t)((N + 1) & ~1) : 0;
+}
 
 int create_from_networks(std::vector<Network> &nets, const mf_solver_params *params,
                          mf_batch **out) {
@@ -603,19 +616,25 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.tol = b->p.tolerance;
   std::vector<std::pair<int, int>> groups;
   std::vector<size_t> glds;
+  std::vector<int64_t> ggs;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
   for (int i = 0; i < n;) {
     int v = b->plan[prec][S.rve[order[i]]];
     int j = i;
     size_t L = 0;
+    int maxN = 0;
     while (j < n && b->plan[prec][S.rve[order[j]]] == v) {
       L = std::max(L, b->lds[prec][S.rve[order[j]]]);
+      maxN = std::max(maxN, b->host[S.rve[order[j]]].N);
       ++j;
     }
     groups.push_back({v, j - i});
     glds.push_back(L);
+    ggs.push_back(group_gstride(vs[v], maxN));
     i = j;
   }
-  return launch_groups(b, a, b->order.p, groups, glds);
+  return launch_groups(b, a, b->order.p, groups, glds, ggs);
 }
 
 void fill_args(mf_batch *b, SolveArgs &a) {
@@ -663,17 +682,27 @@ int build_identity_tables(mf_batch *b) {
     const std::vector<int32_t> order = launch_order(b, prec, R, [](int i) { return i; });
     b->id_groups[prec].clear();
     b->id_group_lds[prec].clear();
+    b->id_group_gs[prec].clear();
+    int nv = 0;
+    const Variant *vs = variants(&nv);
+    size_t gtotal = 0;
     for (int i = 0; i < R;) {
       int v = b->plan[prec][order[i]], j = i;
       size_t L = 0;
+      int maxN = 0;
       while (j < R && b->plan[prec][order[j]] == v) {
         L = std::max(L, b->lds[prec][order[j]]);
+        maxN = std::max(maxN, b->host[order[j]].N);
         ++j;
       }
       b->id_groups[prec].push_back({v, j - i});
       b->id_group_lds[prec].push_back(L);
+      b->id_group_gs[prec].push_back(group_gstride(vs[v], maxN));
+      gtotal += (size_t)(j - i) * (size_t)b->id_group_gs[prec].back();
       i = j;
     }
+    // allocated here, so a captured run launch never allocates
+    MF_HIP(b->gstate.ensure(gtotal));
     MF_HIP(b->id_order[prec].upload(order, s));
     // flags: [0..R) update=false, [R..2R) update=true
     std::vector<int32_t> fl(2 * (size_t)R);
@@ -693,10 +722,21 @@ int build_identity_tables(mf_batch *b) {
 // streams, all after ev0 and joined back before ev1.
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
-                  const std::vector<size_t> &lds) {
+                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride) {
   int nv = 0;
   const Variant *vs = variants(&nv);
   hipStream_t s = b->stream;
+  size_t gtotal = 0;
+  for (size_t gi = 0; gi < groups.size(); ++gi)
+    gtotal += (size_t)groups[gi].second * (size_t)gstride[gi];
+  if (gtotal > b->gstate.n) {
+    if (b->capturing) {
+      set_error("global-state scratch would grow inside a captured launch");
+      return MF_ERR_STATE;
+    }
+    MF_HIP(b->gstate.ensure(gtotal));
+  }
+  size_t goff = 0;
   // under capture the fork uses its own event and the timing events stay
   // outside the graph
   hipEvent_t fork = b->capturing ? b->ev_fork : b->ev0;
@@ -712,6 +752,9 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
       used[k] = true;
     }
     a.order = order + off;
+    a.gstate = gstride[gi] ? b->gstate.p + goff : nullptr;
+    a.gstride = gstride[gi];
+    goff += (size_t)groups[gi].second * (size_t)gstride[gi];
     hipError_t e = launch_solve(vs[groups[gi].first], a, groups[gi].second, lds[gi], st);
     if (e != hipSuccess) {
       set_error(std::string("solve launch: ") + hipGetErrorString(e));
@@ -795,7 +838,7 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
   }
   {
     int e = launch_groups(b, a, b->id_order[prec].p, b->id_groups[prec],
-                          b->id_group_lds[prec]);
+                          b->id_group_lds[prec], b->id_group_gs[prec]);
     if (e != MF_OK) return e;
   }
   if (!ramp) b->at_reference = false;  // relax runs keep the trial x (Q5)
@@ -1185,9 +1228,12 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
       b->tuned[key] = c->second;
       continue;
     }
+    // the global-state variant only where nothing else fits
     std::vector<int> cand;
     for (int i = 0; i < nv; ++i)
-      if (fits_all(i)) cand.push_back(i);
+      if (!(vs[i].flags & kGlobalState) && fits_all(i)) cand.push_back(i);
+    for (int i = 0; cand.empty() && i < nv; ++i)
+      if ((vs[i].flags & kGlobalState) && fits_all(i)) cand.push_back(i);
     if (cand.empty()) continue;
     int best = cand[0];
     double best_us = 0.0;

@@ -47,6 +47,9 @@ struct SolveArgs {
   int32_t pad_;
   double energy_eps;        // > 0: energy balance check (energy variants)
   double *energy;           // 4 per solve (energy variants), may be NULL
+  // global-state variants: per-solve scratch at gstate + block * gstride
+  double *gstate;
+  int64_t gstride;          // doubles per solve
 };
 
 // A kernel variant: threads per solve, free-node slots per thread and the
@@ -60,12 +63,14 @@ enum : int32_t {
   kMix2 = 32,       // at most one linear and one nonlinear reaction (FP32c registers)
   kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
   kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
+  kGlobalState = 8192, // FP64 state in global memory: any RVE size
 };
 // Stride (float4 entries) of the two position buffers of the register
 // kernel: buffer 1 starts 32 KiB into LDS, a compile-time offset the LDS
 // loads take as an immediate; RVEs of up to this many nodes.
 constexpr int kRegBufStride = 2048;
-// Fibers per thread of a fiber-parallel variant (flags bits 8-11).
+// Fibers per thread of a fiber-parallel variant (flags bits 8-11; the
+// other flags stay clear of them).
 constexpr int kFplShift = 8;
 inline int variant_fpl(int32_t flags) { return (flags >> kFplShift) & 15; }
 

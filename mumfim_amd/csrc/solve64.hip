@@ -488,6 +488,299 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
   }
 }
 
+// ============================================================================
+// FP64 kernel with the state in global memory (any RVE size)
+// ============================================================================
+// The RVEs no LDS/register kernel holds (more free nodes than lanes, or
+// positions beyond LDS).  Per node the arithmetic and its order are
+// solve64's; positions (double-buffered), velocities and forces live in a
+// per-solve global scratch, a.gstate + blockIdx.x * a.gstride doubles laid
+// out [2][3][Np] positions, [3][Np] velocities, [3][Np] forces, and each
+// thread loops over its free nodes r = tid, tid + B, ...  The workgroup
+// barrier orders its waves' global accesses (one CU).  LDS keeps the
+// reduction scratch, the boundary state and the reaction table.
+template <int B>
+__global__ __launch_bounds__(B) void solve64g(SolveArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const double eta_r = a.eta_rve ? a.eta_rve[rv] : a.eta;
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int64_t nb = R.node_off;
+
+  // LDS carve (mirrors lds_bytes)
+  size_t off = 0;
+  double *red = reinterpret_cast<double *>(smem);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * 6 * Nbp;
+  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
+  // global state of this solve
+  double *pos = a.gstate + (size_t)blockIdx.x * (size_t)a.gstride;  // [2][3][Np]
+  double *gv = pos + 6 * (size_t)Np;                                // [3][Np]
+  double *gf = gv + 3 * (size_t)Np;                                 // [3][Np]
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  const bool lin_only =
+      R.n_react == 1 && a.reacts[R.react_off].type == MF_REACTION_LINEAR;
+  const Lin64 lin{a.reacts[R.react_off].E, a.reacts[R.react_off].A};
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  // ---- initial state (as solve64) -------------------------------------------
+  for (int r = tid; r < N; r += B) {
+    const int64_t g = nb + r;
+    double xp[3], up[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+    double inc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double u = ramp ? up[k] : inc[k] + up[k];
+      const double x = a.X[3 * g + k] + u;
+      pos[k * Np + r] = x;
+      pos[3 * Np + k * Np + r] = x;
+      if (r < Nf) gv[k * Np + r] = 0.0;
+    }
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = pos[k * Np + r];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+
+  const double c_sound = R.c_sound;
+  int cur = 0;
+  // internal force at node r (free or boundary) from the positions in buffer
+  // `buf`: the node's fibers in increasing fiber index
+  auto node_force = [&](int r, int buf, double (&f)[3]) {
+    const double *P = pos + buf * 3 * Np;
+    const double x0 = P[r], x1 = P[Np + r], x2 = P[2 * Np + r];
+    const int base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
+    const int dg = a.deg[nb + r];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const double l0 = gl0[idx];
+      const int w = wv & 0xFFFFFF;
+      const double dx = P[w] - x0;
+      const double dy = P[Np + w] - x1;
+      const double dz = P[2 * Np + w] - x2;
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      const double t = frc / l;
+      fx = fx - dx * t;
+      fy = fy - dy * t;
+      fz = fz - dz * t;
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  auto min_length = [&](int buf) {
+    const double *P = pos + buf * 3 * Np;
+    double lmin = 1.0e300;
+    const int32_t *fb = a.fib + 2 * R.fib_off;
+    for (int e = tid; e < M; e += B) {
+      const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+      double x1 = P[n2] - P[n1];
+      double x2 = P[Np + n2] - P[Np + n1];
+      double x3 = P[2 * Np + n2] - P[2 * Np + n1];
+      double l = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+      lmin = l < lmin ? l : lmin;
+    }
+    return lmin;
+  };
+
+  // ---- RunInitialization (TeamOuterLoop.h:215-289) ---------------------------
+  double resid_local = 0.0;
+  for (int r = tid; r < Nf; r += B) {
+    double fint[3];
+    node_force(r, cur, fint);
+    const double mr = a.mass[nb + r];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double mass = mr * eta_r;
+      double fdamp = mass * gv[d * Np + r];
+      gf[d * Np + r] = -fint[d] - fdamp;
+    }
+  }
+  double lmin = min_length(cur);
+  block_min_sum<B>(lmin, resid_local, red);
+  double dt_crit = lmin / c_sound;
+
+  // ---- main loop (TeamOuterLoop.h:340-371) -----------------------------------
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  for (;;) {
+    const double h = a.crit * dt_crit * 0.5;
+    const int nxt = cur ^ 1;
+    const double *Pc = pos + cur * 3 * Np;
+    double *Pn = pos + nxt * 3 * Np;
+    // updates(): half kick + drift of free nodes (TeamOuterLoop.h:157-175)
+    for (int r = tid; r < Nf; r += B) {
+      const double mr = a.mass[nb + r];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double vl = gv[d * Np + r] + h * gf[d * Np + r] / mr;
+        gv[d * Np + r] = vl;
+        Pn[d * Np + r] = Pc[d * Np + r] + 2 * h * vl;
+      }
+    }
+    if (ramp && tid < Nb) {
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          Pn[d * Np + Nf + b] = bst[d * Nbp + b] + alpha * bst[(3 + d) * Nbp + b];
+      }
+    }
+    __syncthreads();
+    cur = nxt;
+    const bool refresh = (step % freq) == 0;
+    double rsum = 0.0;
+    for (int r = tid; r < Nf; r += B) {
+      double fint[3];
+      node_force(r, cur, fint);
+      const double mr = a.mass[nb + r];
+      double rn = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double mass = mr * eta_r;
+        double fdamp = mass * gv[d * Np + r];
+        double lres = -fint[d];
+        gf[d * Np + r] = lres - fdamp;
+        rn += lres * lres;
+      }
+      rsum += rn;
+    }
+    if (refresh) {
+      double lm = min_length(cur);
+      block_min_sum<B>(lm, rsum, red);
+      dt_crit = lm / c_sound;
+      residual = sqrt(rsum);
+    }
+    // updateAccelVels(): second half kick (TeamOuterLoop.h:144-156)
+    for (int r = tid; r < Nf; r += B) {
+      const double mr = a.mass[nb + r];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gv[d * Np + r] += h * gf[d * Np + r] / mr;
+    }
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // ---- FinalizeData + computeCauchyStress (as solve64) ----------------------
+  const double *P = pos + cur * 3 * Np;
+  __syncthreads();
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const int flags = a.solve_flags[sid];
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    const int r = Nf + b;
+    double xb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xb[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const int base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
+    const int dg = a.deg[nb + r];
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const double l0 = gl0[idx];
+      const int w = wv & 0xFFFFFF;
+      const double dx = P[w] - xb[0];
+      const double dy = P[Np + w] - xb[1];
+      const double dz = P[2 * Np + w] - xb[2];
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      const double t = frc / l;
+      f1 = f1 - dx * t;
+      f2 = f2 - dy * t;
+      f3 = f3 - dz * t;
+    }
+    st[0] += xb[0] * f1;
+    st[1] += xb[1] * f2;
+    st[2] += xb[2] * f3;
+    st[3] += 0.5 * (xb[1] * f3 + xb[2] * f2);
+    st[4] += 0.5 * (xb[0] * f3 + xb[2] * f1);
+    st[5] += 0.5 * (xb[0] * f2 + xb[1] * f1);
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * (nb + r) + d] = xb[d];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * (nb + r) + d] = xb[d] - a.X[3 * (nb + r) + d];
+  }
+  // free nodes: u = x - X (TeamOuterLoop.h:427-432)
+  for (int r = tid; r < Nf; r += B) {
+    const int64_t g = nb + r;
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * g + d] = P[d * Np + r];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * g + d] = P[d * Np + r] - a.X[3 * g + d];
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !isfinite(v);
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
 }  // namespace kern
 
 hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
@@ -506,6 +799,14 @@ hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
+  if (v.block == 1024 && v.flags == kGlobalState) {
+    auto k = kern::solve64g<1024>;
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kMaxLds);
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(1024), lds, s, a);
+    return hipGetLastError();
+  }
   return hipErrorInvalidValue;
 }
 

@@ -384,6 +384,41 @@ def test_fiber_kernels_match_oracle():
     assert np.abs(u_f - u_n).max() <= 1e-4 * np.abs(u_n).max()
 
 
+def test_large_rve_global_state_kernel():
+    """An RVE with more free nodes (~5400) than the register and LDS kernels
+    have lanes takes the FP64 kernel with the state in global memory: it
+    matches the oracle like the other FP64 kernels, and in an FP32c batch the
+    same RVE falls back to it (bitwise the FP64 result) while a small RVE
+    keeps an FP32c kernel."""
+    say("case: 6000-node RVE, global-state FP64 kernel")
+    big = mf.generate_network(seed=7, n_nodes=6000, nonlinear_fraction=0.5)
+    small = mf.generate_network(seed=8, n_nodes=400, nonlinear_fraction=0.5)
+    common = dict(n_steps=400, reaction_policy=mf.MF_REACT_PER_FIBER)
+    F = strain_F(12, 2, 0.05)
+    g64 = mf.BatchedFiberRVEAnalysisExplicit([big, small], ramp_params(**common))
+    _, p64 = g64.plan()
+    say(f"  FP64 plan {p64[:, :3].tolist()}")
+    assert p64[0, 2] == 8192 and p64[1, 2] != 8192, p64
+    s64, st = g64.run(F, return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
+    s_o = Oracle([big, small], ramp_params(**common)).run(F)
+    err = rel_err(s64, s_o)
+    say(f"  FP64 vs oracle {err:.2e}")
+    assert err <= FP64_TOL
+    x64, u64, _ = g64.get_state()
+    g32 = mf.BatchedFiberRVEAnalysisExplicit([big, small],
+                                             ramp_params(precision=mf.MF_PREC_FP32C, **common))
+    _, p32 = g32.plan()
+    say(f"  FP32c plan {p32[:, :3].tolist()}")
+    assert p32[0, 2] == 8192 and p32[1, 2] != 8192, p32
+    s32 = g32.run(F)
+    assert np.array_equal(s32[0], s64[0])
+    assert rel_err(s32[1:], s_o[1:]) <= FP32C_TOL
+    x32, u32, _ = g32.get_state()
+    n0 = big.n_nodes
+    assert np.array_equal(u32[:n0], u64[:n0])
+
+
 def test_fp32c_families_relax_mode_and_step_cap():
     """Reference (relax) mode through each FP32c kernel family: node-centric
     (register lists), fiber-parallel (300 nodes) and fiber-streamed (1300

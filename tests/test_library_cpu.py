@@ -104,3 +104,19 @@ def test_batch_create_rejects_bad_input_before_touching_the_gpu():
     bad = mf.Network(net.coords, np.array([[0, 0]], np.int32), net.boundary, net.reactions)
     with pytest.raises(mf.MfError):
         mf.BatchedFiberRVEAnalysisExplicit([bad])
+
+
+def test_planner_takes_any_rve_size():
+    """An RVE with more free nodes than any register/LDS kernel holds (12000
+    nodes) is planned onto the FP64 global-state kernel in both precisions,
+    so creation gets past planning (here it stops at the missing device);
+    with FP64 energy diagnostics, which that kernel does not track, it is
+    refused by the planner."""
+    net = mf.generate_network(seed=3, n_nodes=12000, nonlinear_fraction=0.5)
+    for prec in (mf.MF_PREC_FP64, mf.MF_PREC_FP32C):
+        try:
+            mf.BatchedFiberRVEAnalysisExplicit([net], mf.default_params(precision=prec)).close()
+        except mf.MfError as e:
+            assert "fits no kernel variant" not in str(e), e
+    with pytest.raises(mf.MfError, match="fits no kernel variant"):
+        mf.BatchedFiberRVEAnalysisExplicit([net], mf.default_params(energy_diagnostics=1))
