@@ -341,8 +341,14 @@ int make_plan(mf_batch *b, int prec) {
       // 256 threads x 2 linear nodes per thread (three solves per CU) beats
       // one node per thread at 512+ threads: 9% on 500-node linear RVEs
       const bool reg256x2 = regs && v.block == 256 && v.npt == 2;
-      // global-state FP64 last: only where nothing else fits
+      // global-state FP64 last: only where nothing else fits.  FP64 with each
+      // fiber evaluated once before the node-centric FP64 kernels for RVEs
+      // with toe-law fibers (11% on 1500-node RVEs), after them for linear
+      // ones, whose cheap force does not pay for the extra phase (13% slower
+      // on 500-node RVEs)
       long key = (v.flags & kGlobalState) ? 2000000000L
+                 : (v.flags & kFiberOnce)
+                     ? (lin ? 1500000000L : 500000000L) + (long)v.npt * 10000000 + (long)v.block * 1000
                  : fpar ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
                  : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
                  : reg256x2 ? 10000000L + 5000
@@ -388,6 +394,9 @@ int upload_batch(mf_batch *b) {
     d.n_slices = (int)p.slice_base.size();
     d.ell_free_entries = p.ell_free_entries;
     d.n_split = p.n_split;
+    d.m_free = 0;
+    for (int e = 0; e < p.M; ++e)
+      if (p.fstream[4 * (size_t)e + 2] >= 0) d.m_free = e + 1;
     d.react_off = (int)rx.size();
     d.n_react = (int)p.reactions.size();
     d.c_sound = p.c_sound;

@@ -176,6 +176,13 @@ static const Variant kVariants[] = {
   {MF_PREC_FP32C, B, NPL, kFiberStream | FL | ((FPL) << kFplShift)},
     MF_FIBER_STREAM_LIST(MF_FSTR_ENTRY)
 #undef MF_FSTR_ENTRY
+    // FP64, each fiber evaluated once per step: (threads, free nodes per thread)
+    {MF_PREC_FP64, 256, 1, kFiberOnce},
+    {MF_PREC_FP64, 256, 2, kFiberOnce},
+    {MF_PREC_FP64, 512, 1, kFiberOnce},
+    {MF_PREC_FP64, 512, 2, kFiberOnce},
+    {MF_PREC_FP64, 1024, 1, kFiberOnce},
+    {MF_PREC_FP64, 1024, 2, kFiberOnce},
     // FP64 with the state in global memory: RVEs no other variant holds
     {MF_PREC_FP64, 1024, 1, kGlobalState},
 };
@@ -189,6 +196,13 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
   const size_t Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
   const size_t NW = v.block / kern::kWave;
   size_t b = 0;
+  if (v.flags & kFiberOnce) {
+    // double-buffered positions, t per fiber (M bounds the fibers with a
+    // free end), scratch, boundary state, reactions
+    return sizeof(double) * 6 * Np + sizeof(double) * (((size_t)M + 1) & ~size_t(1)) +
+           sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp +
+           ((sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15));
+  }
   if (v.flags & kGlobalState) {
     // scratch, boundary state, reactions (positions etc. in global memory)
     return sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp +

@@ -126,7 +126,7 @@ struct RveDev {
   int32_t ell_free_entries;
   int32_t react_off;
   int32_t n_react;
-  int32_t pad_;
+  int32_t m_free;     // fibers with a free end (the fiber-stream prefix)
   double c_sound, scale;
   // Reference coordinates in fixed point for the FP32 kernel: X_q =
   // rint(X * xq_scale) fits int32; xq_scale is a power of two.

@@ -64,6 +64,7 @@ enum : int32_t {
   kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
   kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
   kGlobalState = 8192, // FP64 state in global memory: any RVE size
+  kFiberOnce = 16384,  // FP64, t = f/l once per fiber, nodes gather by reference
 };
 // Stride (float4 entries) of the two position buffers of the register
 // kernel: buffer 1 starts 32 KiB into LDS, a compile-time offset the LDS

@@ -489,6 +489,361 @@ __global__ __launch_bounds__(B) void solve64(SolveArgs a) {
 }
 
 // ============================================================================
+// FP64 kernel, each fiber evaluated once per step
+// ============================================================================
+// Bitwise the result of solve64: at end b the difference vector is exactly
+// the negation of end a's (x_a - x_b = -(x_b - x_a) in IEEE arithmetic), so
+// the length, the force and t = f/l of a fiber are the same numbers at both
+// ends.  A fiber phase computes t once per fiber with a free end (fiber
+// stream order, l0 and reaction from end a's ELL entry) into LDS; each free
+// node then recomputes its difference vectors and subtracts d t in its ELL
+// order, taking t by the entry's fiber reference, which are solve64's
+// operations and order without the second square root and divisions.
+// Positions double-buffered in LDS; two barriers per step (positions
+// published, t published).
+template <int B, int NPT>
+__global__ __launch_bounds__(B) void solve64f(SolveArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NW = B / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int rv = a.solve_rve[sid];
+  const RveDev R = a.rves[rv];
+  const double eta_r = a.eta_rve ? a.eta_rve[rv] : a.eta;
+  const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M, Mf = R.m_free;
+  const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
+  const int64_t nb = R.node_off;
+
+  // LDS carve (mirrors lds_bytes)
+  size_t off = 0;
+  double *pos = reinterpret_cast<double *>(smem);  // [2][3][Np]
+  off += sizeof(double) * 6 * Np;
+  double *trec = reinterpret_cast<double *>(smem + off);  // [Mf] t per fiber
+  off += sizeof(double) * ((Mf + 1) & ~1);
+  double *red = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * NW * kRedStride;
+  double *bst = reinterpret_cast<double *>(smem + off);
+  off += sizeof(double) * 6 * Nbp;
+  ReactDev *rxs = reinterpret_cast<ReactDev *>(smem + off);
+
+  const int32_t *gw = a.ell_w + R.ell_off;
+  const double *gl0 = a.ell_l0 + R.ell_off;
+  const int32_t *gfr = a.ell_fref + R.ell_off;
+  const int32_t *fst = a.fstream + 4 * R.fib_off;
+  if (tid < R.n_react) rxs[tid] = a.reacts[R.react_off + tid];
+  const bool lin_only =
+      R.n_react == 1 && a.reacts[R.react_off].type == MF_REACTION_LINEAR;
+  const Lin64 lin{a.reacts[R.react_off].E, a.reacts[R.react_off].A};
+
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.solve_F[9 * (int64_t)sid + i];
+  const bool ramp = a.mode == MF_MODE_RAMP;
+
+  // ---- initial state (as solve64) -------------------------------------------
+  double xr[NPT][3], vr[NPT][3], fr[NPT][3], mr[NPT];
+  int degr[NPT], Dr[NPT], baser[NPT];
+  auto init_node = [&](int r, double (&x)[3]) {
+    const int64_t g = nb + r;
+    double xp[3], up[3], u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = a.x_in[3 * g + d];
+      up[d] = a.u_in[3 * g + d];
+    }
+    double inc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+               (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+               (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u[k] = ramp ? up[k] : inc[k] + up[k];
+      x[k] = a.X[3 * g + k] + u[k];
+      pos[k * Np + r] = x[k];
+      pos[3 * Np + k * Np + r] = x[k];
+    }
+    if (r >= Nf) {
+      const int b = r - Nf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bst[k * Nbp + b] = x[k];
+        bst[(3 + k) * Nbp + b] = ramp ? inc[k] : 0.0;
+      }
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    const int r = s * B + tid;
+    double x[3] = {0, 0, 0};
+    if (r < N) init_node(r, x);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xr[s][d] = x[d];
+      vr[s][d] = 0.0;
+      fr[s][d] = 0.0;
+    }
+    degr[s] = 0;
+    Dr[s] = 0;
+    baser[s] = 0;
+    mr[s] = 1.0;
+    if (r < Nf) {
+      mr[s] = a.mass[nb + r];
+      degr[s] = a.deg[nb + r];
+      const int k = r / kWave;
+      Dr[s] = a.slice_D[R.slice_off + k];
+      baser[s] = a.slice_base[R.slice_off + k] + lane;
+    }
+  }
+  for (int r = NPT * B + tid; r < N; r += B) {
+    double x[3];
+    init_node(r, x);
+  }
+  __syncthreads();
+
+  const double c_sound = R.c_sound;
+  int cur = 0;
+  // t = f/l of every fiber with a free end, at the positions in buffer `buf`,
+  // with end a's view of the fiber (d = x_b - x_a)
+  auto fiber_phase = [&](int buf) {
+    const double *P = pos + buf * 3 * Np;
+    for (int e = tid; e < Mf; e += B) {
+      const int ra = fst[4 * e], rb = fst[4 * e + 1], sa = fst[4 * e + 2];
+      const int32_t wv = gw[sa];
+      const double l0 = gl0[sa];
+      const double dx = P[rb] - P[ra];
+      const double dy = P[Np + rb] - P[Np + ra];
+      const double dz = P[2 * Np + rb] - P[2 * Np + ra];
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      trec[e] = frc / l;
+    }
+  };
+  // internal force of a free node: its fibers in ELL order, t by reference
+  auto node_force = [&](int s, int buf, double (&f)[3]) {
+    const double *P = pos + buf * 3 * Np;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int D = __builtin_amdgcn_readfirstlane(Dr[s]);
+    int32_t wv_n = D > 0 ? gw[baser[s]] : 0;
+    int32_t fr_n = D > 0 ? gfr[baser[s]] : 0;
+    for (int j = 0; j < D; ++j) {
+      const int32_t wv = wv_n, frf = fr_n;
+      if (j + 1 < D) {
+        wv_n = gw[baser[s] + (j + 1) * kWave];
+        fr_n = gfr[baser[s] + (j + 1) * kWave];
+      }
+      if (j < degr[s]) {
+        const int w = wv & 0xFFFFFF;
+        const double t = trec[(uint32_t)(frf & 0x7FFFFFFF) / 12u];
+        const double dx = P[w] - xr[s][0];
+        const double dy = P[Np + w] - xr[s][1];
+        const double dz = P[2 * Np + w] - xr[s][2];
+        fx = fx - dx * t;
+        fy = fy - dy * t;
+        fz = fz - dz * t;
+      }
+    }
+    f[0] = fx;
+    f[1] = fy;
+    f[2] = fz;
+  };
+  auto min_length = [&](int buf) {
+    const double *P = pos + buf * 3 * Np;
+    double lmin = 1.0e300;
+    const int32_t *fb = a.fib + 2 * R.fib_off;
+    for (int e = tid; e < M; e += B) {
+      const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
+      double x1 = P[n2] - P[n1];
+      double x2 = P[Np + n2] - P[Np + n1];
+      double x3 = P[2 * Np + n2] - P[2 * Np + n1];
+      double l = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+      lmin = l < lmin ? l : lmin;
+    }
+    return lmin;
+  };
+
+  // ---- RunInitialization (TeamOuterLoop.h:215-289) ---------------------------
+  double fint[NPT][3];
+  double resid_local = 0.0;
+  fiber_phase(cur);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    if (s * B + tid < Nf) {
+      node_force(s, cur, fint[s]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double mass = mr[s] * eta_r;
+        double fdamp = mass * vr[s][d];
+        fr[s][d] = -fint[s][d] - fdamp;
+      }
+    }
+  }
+  double lmin = min_length(cur);
+  block_min_sum<B>(lmin, resid_local, red);
+  double dt_crit = lmin / c_sound;
+
+  // ---- main loop (TeamOuterLoop.h:340-371) -----------------------------------
+  int step = 0;
+  double residual = 10.0;
+  int status = MF_STATUS_OK;
+  const int freq = a.freq;
+  for (;;) {
+    const double h = a.crit * dt_crit * 0.5;
+    const int nxt = cur ^ 1;
+    double *Pn = pos + nxt * 3 * Np;
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      const int r = s * B + tid;
+      if (r < Nf) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double vl = vr[s][d] + h * fr[s][d] / mr[s];
+          vr[s][d] = vl;
+          xr[s][d] = xr[s][d] + 2 * h * vl;
+          Pn[d * Np + r] = xr[s][d];
+        }
+      }
+    }
+    if (ramp && tid < Nb) {
+      const double alpha = ramp_alpha(step + 1, a.n_steps, a.amplitude);
+      for (int b = tid; b < Nb; b += B) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          Pn[d * Np + Nf + b] = bst[d * Nbp + b] + alpha * bst[(3 + d) * Nbp + b];
+      }
+    }
+    __syncthreads();  // positions published; every t read of the last step done
+    cur = nxt;
+    fiber_phase(cur);
+    __syncthreads();  // t published
+    const bool refresh = (step % freq) == 0;
+    double rsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      if (s * B + tid < Nf) {
+        node_force(s, cur, fint[s]);
+        double rn = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double mass = mr[s] * eta_r;
+          double fdamp = mass * vr[s][d];
+          double lres = -fint[s][d];
+          fr[s][d] = lres - fdamp;
+          rn += lres * lres;
+        }
+        rsum += rn;
+      }
+    }
+    if (refresh) {
+      double lm = min_length(cur);
+      block_min_sum<B>(lm, rsum, red);
+      dt_crit = lm / c_sound;
+      residual = sqrt(rsum);
+    }
+#pragma unroll
+    for (int s = 0; s < NPT; ++s) {
+      if (s * B + tid < Nf) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vr[s][d] += h * fr[s][d] / mr[s];
+      }
+    }
+    ++step;
+    if (ramp) {
+      if (step >= a.n_steps) break;
+    } else {
+      if (!(residual > a.tol)) break;
+      if (step >= a.max_steps) {
+        status = MF_STATUS_STEP_CAP;
+        break;
+      }
+    }
+  }
+  if (residual != residual) status = MF_STATUS_NAN;
+
+  // ---- FinalizeData + computeCauchyStress (as solve64) ----------------------
+  const double *P = pos + cur * 3 * Np;
+  __syncthreads();
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  const int flags = a.solve_flags[sid];
+  const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
+  for (int b = tid; b < Nb; b += B) {
+    const int r = Nf + b;
+    double xb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xb[d] = bst[d * Nbp + b] + alpha_end * bst[(3 + d) * Nbp + b];
+    const int base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
+    const int dg = a.deg[nb + r];
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int j = 0; j < dg; ++j) {
+      const int idx = base + j * kWave;
+      const int32_t wv = gw[idx];
+      const double l0 = gl0[idx];
+      const int w = wv & 0xFFFFFF;
+      const double dx = P[w] - xb[0];
+      const double dy = P[Np + w] - xb[1];
+      const double dz = P[2 * Np + w] - xb[2];
+      const double l = sqrt(dx * dx + dy * dy + dz * dz);
+      double frc;
+      if (lin_only) {
+        double lr = l / l0;
+        double gs = 1.0 / 2.0 * (lr * lr - 1);
+        frc = lr * lin.E * lin.A * gs;
+      } else {
+        frc = react64(rxs[(uint32_t)wv >> 24], l0, l);
+      }
+      const double t = frc / l;
+      f1 = f1 - dx * t;
+      f2 = f2 - dy * t;
+      f3 = f3 - dz * t;
+    }
+    st[0] += xb[0] * f1;
+    st[1] += xb[1] * f2;
+    st[2] += xb[2] * f3;
+    st[3] += 0.5 * (xb[1] * f3 + xb[2] * f2);
+    st[4] += 0.5 * (xb[0] * f3 + xb[2] * f1);
+    st[5] += 0.5 * (xb[0] * f2 + xb[1] * f1);
+    if (flags & kWriteX)
+      for (int d = 0; d < 3; ++d) a.x_out[3 * (nb + r) + d] = xb[d];
+    if (flags & kWriteU)
+      for (int d = 0; d < 3; ++d) a.u_out[3 * (nb + r) + d] = xb[d] - a.X[3 * (nb + r) + d];
+  }
+#pragma unroll
+  for (int s = 0; s < NPT; ++s) {
+    const int r = s * B + tid;
+    if (r < Nf) {
+      const int64_t g = nb + r;
+      if (flags & kWriteX)
+        for (int d = 0; d < 3; ++d) a.x_out[3 * g + d] = xr[s][d];
+      if (flags & kWriteU)
+        for (int d = 0; d < 3; ++d) a.u_out[3 * g + d] = xr[s][d] - a.X[3 * g + d];
+    }
+  }
+  block_sum<B, 6>(st, red);
+  if (tid == 0) {
+    const double sv = R.scale / a.V[rv];
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) {
+      double v = st[i] * sv;
+      a.sigma[6 * (int64_t)sid + i] = v;
+      bad = bad || !isfinite(v);
+    }
+    if (bad && status == MF_STATUS_OK) status = MF_STATUS_NAN;
+    a.status[sid] = status;
+    a.steps[sid] = step;
+  }
+}
+
+// ============================================================================
 // FP64 kernel with the state in global memory (any RVE size)
 // ============================================================================
 // The RVEs no LDS/register kernel holds (more free nodes than lanes, or
@@ -799,6 +1154,23 @@ hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
   }
   MF_VARIANT_LIST(MF_VARIANT_LAUNCH)
 #undef MF_VARIANT_LAUNCH
+#define MF_FONCE_LAUNCH(B, N)                                                 \
+  if (v.block == B && v.npt == N && v.flags == kFiberOnce) {                 \
+    auto k = kern::solve64f<B, N>;                                           \
+    if (lds > 64 * 1024)                                                     \
+      (void)hipFuncSetAttribute((const void *)k,                             \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)kMaxLds);                               \
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(B), lds, s, a);               \
+    return hipGetLastError();                                                \
+  }
+  MF_FONCE_LAUNCH(256, 1)
+  MF_FONCE_LAUNCH(256, 2)
+  MF_FONCE_LAUNCH(512, 1)
+  MF_FONCE_LAUNCH(512, 2)
+  MF_FONCE_LAUNCH(1024, 1)
+  MF_FONCE_LAUNCH(1024, 2)
+#undef MF_FONCE_LAUNCH
   if (v.block == 1024 && v.flags == kGlobalState) {
     auto k = kern::solve64g<1024>;
     if (lds > 64 * 1024)
