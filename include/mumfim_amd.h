@@ -236,8 +236,9 @@ int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
  * 4 incidence lists in registers, 8 linear-only, 16 energy diagnostics,
  * 32 one linear and one nonlinear reaction, 64 fiber-parallel, 128
  * fiber-streamed (bits 8-11: fibers per thread), 8192 FP64 with the state in
- * global memory (RVEs no other variant holds, in either precision).  Returns
- * the number of distinct launch groups. */
+ * global memory (RVEs no other variant holds, in either precision), 16384
+ * FP64 evaluating each fiber once (bitwise the node-centric FP64 result).
+ * Returns the number of distinct launch groups. */
 int mf_batch_plan(mf_batch *b, int32_t *plan);
 
 /* ---- device memory (no reference counterpart) ----------------------------- */
