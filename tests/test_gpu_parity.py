@@ -109,6 +109,35 @@ def test_network27_tangent_fp64_matches_oracle(golden, net27):
     assert rel_err(Cc_g, C_g) < 5e-3
 
 
+def test_relax_toe_law_fp64_fiber_once_matches_oracle():
+    """Reference (relax) mode on toe-law RVEs through the FP64 kernel that
+    evaluates each fiber once per step: stress within the FP64 bar of the
+    oracle and the committed state equal to rounding, like the node-centric
+    FP64 kernel (whose results it reproduces bitwise)."""
+    say("case: relax, toe law, FP64 fiber-once kernel")
+    nets = [mf.generate_network(seed=23 + i, n_nodes=n, E=1e8, radius=5e-5,
+                                nonlinear_fraction=0.5) for i, n in enumerate([200, 300])]
+    F = strain_F(13, 2, 0.01)
+    p = mf.default_params(reaction_policy=mf.MF_REACT_PER_FIBER)
+    g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
+    _, plan = g.plan()
+    say(f"  plan {plan[:, :3].tolist()}")
+    assert all(int(f) == 16384 for f in plan[:, 2]), plan
+    s_g, st = g.run(F, return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
+    o = Oracle(nets, p)
+    s_o = o.run(F)
+    err = rel_err(s_g, s_o)
+    say(f"  sigma rel err {err:.3e}")
+    assert err <= FP64_TOL
+    xo, uo, _ = o.get_state()
+    xg, ug, _ = g.get_state()
+    assert np.abs(xg - xo).max() <= 1e-14 and np.abs(ug - uo).max() <= 1e-14
+    # forward tangent (chained trials) as well
+    C_g = g.computeMaterialStiffness()
+    assert rel_err(C_g, o.computeMaterialStiffness()) <= 1e-8
+
+
 def test_relax_multi_rve_broadcast_trial(net27):
     say("case: relax, 3 RVEs, trial with broadcast F")
     nets = [net27, mf.generate_network(seed=5, n_nodes=300, E=1e8, radius=5e-5),
