@@ -544,6 +544,7 @@ void solve32r(SolveArgs a) {
   const bool split = tid < 2 * S;  // slot 0 only
   int rr[NPT];
   bool own[NPT], writer[NPT];
+  // imr: 1/m, or m in the linear mode, whose forces come scaled by 1/m
   float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
   int offs[NPT][kLaneDeg];
   float cf[NPT][kLaneDeg], d0x[NPT][kLaneDeg], d0y[NPT][kLaneDeg], d0z[NPT][kLaneDeg];
@@ -574,6 +575,7 @@ void solve32r(SolveArgs a) {
         ul[sl][k] = (float)(u - (double)uh[sl][k]);
       }
       imr[sl] = (float)(1.0 / a.mass[g]);
+      if (LIN) imr[sl] = (float)a.mass[g];  // linear: forces come scaled by 1/m
       dg = a.deg[g];
       base = a.slice_base[R.slice_off + r / kWave] + (r & (kWave - 1));
       if (sl == 0 && split) {
@@ -597,7 +599,8 @@ void solve32r(SolveArgs a) {
         offs[sl][j] = w * (int)sizeof(float4) | (RM == RM_TABLE ? (wv & 0xFF000000) : 0);
         const double l0 = gl0[idx];
         if (RM == RM_LIN) {
-          cf[sl][j] = (float)(EA_half / (l0 * l0 * l0));
+          // EA/(2 l0^3 m): the gathered force is the acceleration
+          cf[sl][j] = (float)(EA_half / (l0 * l0 * l0 * a.mass[nb + r]));
         } else if (RM == RM_MIX2) {
           const ReactDev &q = a.reacts[R.react_off + ((uint32_t)wv >> 24)];
           cf[sl][j] = q.type == MF_REACTION_NONLINEAR ? -gil[idx]
@@ -731,6 +734,10 @@ void solve32r(SolveArgs a) {
   const float crit_half = (float)(a.crit * 0.5);
   float h_prev = 0.0f;
   float fint[NPT][3];
+  // acceleration of a slot's node: linear forces already carry 1/m
+  auto accel = [&](int sl, int d) __attribute__((always_inline)) {
+    return LIN ? fint[sl][d] : fint[sl][d] * imr[sl];
+  };
   {
     double resid_local = 0.0;
 #pragma unroll
@@ -745,7 +752,7 @@ void solve32r(SolveArgs a) {
       if (own[sl]) {
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-          vr[sl][d] = vr[sl][d] + hk * (-fint[sl][d] * imr[sl] - eta * vr[sl][d]);
+          vr[sl][d] = vr[sl][d] + hk * (-accel(sl, d) - eta * vr[sl][d]);
       }
     h_prev = h;
   }
@@ -794,9 +801,12 @@ void solve32r(SolveArgs a) {
     for (int sl = 0; sl < NPT; ++sl) {
       if (own[sl]) {
         node_force(sl, NXTC, fint[sl], sl == 0 && wave_split);
-        if (refresh && writer[sl])
-          rsum += (double)(fint[sl][0] * fint[sl][0] + fint[sl][1] * fint[sl][1] +
-                           fint[sl][2] * fint[sl][2]);
+        if (refresh && writer[sl]) {
+          // the residual is |f|: linear forces are scaled back by m
+          const float m = LIN ? imr[sl] : 1.0f;
+          const float f0 = fint[sl][0] * m, f1 = fint[sl][1] * m, f2 = fint[sl][2] * m;
+          rsum += (double)(f0 * f0 + f1 * f1 + f2 * f2);
+        }
       }
     }
     float h = h_prev;
@@ -812,7 +822,7 @@ void solve32r(SolveArgs a) {
       if (own[sl]) {
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-          vr[sl][d] = vr[sl][d] + hk * (-fint[sl][d] * imr[sl] - eta * vr[sl][d]);
+          vr[sl][d] = vr[sl][d] + hk * (-accel(sl, d) - eta * vr[sl][d]);
       }
     h_prev = h;
     ++step;
