@@ -94,10 +94,11 @@ struct PackedRVE {
   // consecutive fibers mostly have lane-consecutive a; fibers with two
   // boundary ends come last.
   std::vector<int32_t> fstream;
-  // Per ELL entry: the entry's fiber as a byte offset of its 12-byte force
-  // record in the fiber-streamed kernel (stream position * 12), bit 31 set
-  // where the node is the fiber's end a (it receives -force).  Padding and
-  // boundary-node entries name the zero record at position M.
+  // Per ELL entry: the entry's fiber as its fiber-stream position (the index
+  // of its force record in the fiber-streamed kernel and of its t in the FP64
+  // fiber-once kernel), bit 31 set where the node is the fiber's end a (it
+  // receives -force).  Padding and boundary-node entries name the zero
+  // record at position M.
   std::vector<int32_t> ell_fref;
   // Per fiber in stream order, 4 floats: the reference vector X_b - X_a
   // rounded once to FP32, and the law coefficient (linear: E A / (2 l0^3);

@@ -24,7 +24,7 @@ struct SolveArgs {
   const float *ell_il;      // FP32 kernels: 1/l0
   const int32_t *fib;       // 2 per fiber (kernel ranks)
   const int32_t *fstream;   // 4 per fiber: ranks a, b and their ELL entries
-  const int32_t *ell_fref;  // per ELL entry: force-record byte offset | sign bit
+  const int32_t *ell_fref;  // per ELL entry: fiber record index | sign bit
   const float *fparam;      // 4 per fiber: reference vector, law coefficient
   const ReactDev *reacts;
   const double *x_in, *u_in;

@@ -494,11 +494,11 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
                                               : (float)(0.5 * rq.E * rq.area() / (l0 * l0 * l0));
     }
   }
-  p.ell_fref.assign(total, 12 * M);
+  p.ell_fref.assign(total, M);
   for (int e = 0; e < M; ++e) {
     const int sa = p.fstream[4 * (size_t)e + 2], sb = p.fstream[4 * (size_t)e + 3];
-    if (sa >= 0) p.ell_fref[sa] = (int32_t)(12u * (uint32_t)e | 0x80000000u);
-    if (sb >= 0) p.ell_fref[sb] = 12 * e;
+    if (sa >= 0) p.ell_fref[sa] = (int32_t)((uint32_t)e | 0x80000000u);
+    if (sb >= 0) p.ell_fref[sb] = e;
   }
   return MF_OK;
 }

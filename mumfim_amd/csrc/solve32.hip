@@ -1605,7 +1605,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   // internal force of a free node: -force of fibers where it is end a,
   // +force where it is end b; references prefetched one entry ahead
   auto node_force = [&](int s, float (&f)[3]) {
-    const unsigned char *Fb = reinterpret_cast<const unsigned char *>(ffr);
+    const F3 *Fb = ffr;
     const int32_t *E = gfr + sbn[s] + lane;
     const int D = Dn[s];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -1614,7 +1614,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       for (int j = 0; j < D; ++j) {
         const int32_t fr = nx;
         if (j + 1 < D) nx = E[(j + 1) * kWave];
-        const F3 v = *reinterpret_cast<const F3 *>(Fb + (fr & 0x7FFFFFFF));
+        const F3 v = Fb[fr & 0x7FFFFFFF];
         const float sg = __int_as_float((fr & (int32_t)0x80000000) | 0x3F800000);
         fx = fmaf(sg, v.x, fx);
         fy = fmaf(sg, v.y, fy);

@@ -643,7 +643,7 @@ __global__ __launch_bounds__(B) void solve64f(SolveArgs a) {
       }
       if (j < degr[s]) {
         const int w = wv & 0xFFFFFF;
-        const double t = trec[(uint32_t)(frf & 0x7FFFFFFF) / 12u];
+        const double t = trec[frf & 0x7FFFFFFF];
         const double dx = P[w] - xr[s][0];
         const double dy = P[Np + w] - xr[s][1];
         const double dz = P[2 * Np + w] - xr[s][2];
