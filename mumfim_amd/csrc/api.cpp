@@ -341,6 +341,11 @@ int make_plan(mf_batch *b, int prec) {
       // 256 threads x 2 linear nodes per thread (three solves per CU) beats
       // one node per thread at 512+ threads: 9% on 500-node linear RVEs
       const bool reg256x2 = regs && v.block == 256 && v.npt == 2;
+      // toe-law RVEs needing 640+ threads at one node per thread run faster
+      // fiber-parallel (up to 3 fibers per thread): 8% at 600 and 700 nodes,
+      // 16% at 800, 7% at 1000 (at 500 nodes and 512 threads the register
+      // kernel is 1.45x faster)
+      const bool regbig_nl = regs && !lin && v.block >= 640 && v.npt == 1;
       // global-state FP64 last: only where nothing else fits.  FP64 with each
       // fiber evaluated once before the node-centric FP64 kernels for RVEs
       // with toe-law fibers (11% on 1500-node RVEs), after them for linear
@@ -352,6 +357,7 @@ int make_plan(mf_batch *b, int prec) {
                  : fpar ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
                  : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
                  : reg256x2 ? 10000000L + 5000
+                 : regbig_nl ? 15340000L + mode_rank
                  : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank
                         : 1000000000L + (long)v.npt * 10000000 + (long)v.block * 1000 - v.flags;
       if (best < 0 || key < best_key) {
