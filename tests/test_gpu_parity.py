@@ -446,6 +446,20 @@ def test_large_rve_global_state_kernel():
     x32, u32, _ = g32.get_state()
     n0 = big.n_nodes
     assert np.array_equal(u32[:n0], u64[:n0])
+    # hipGraph replays of trial runs equal the eager trial run: the
+    # global-state scratch exists before the capture
+    Fd = mf.DeviceArray.from_host(F.reshape(2, 9))
+    sd = mf.DeviceArray((2, 6))
+    ge = mf.BatchedFiberRVEAnalysisExplicit([big, small], ramp_params(**common))
+    ge.run_device(Fd, sd)
+    ge.synchronize()
+    s_eager = sd.numpy().copy()
+    gg = mf.BatchedFiberRVEAnalysisExplicit([big, small], ramp_params(**common))
+    gg.set_graph(True)
+    for _ in range(3):
+        gg.run_device(Fd, sd)
+        gg.synchronize()
+        assert np.array_equal(sd.numpy(), s_eager)
 
 
 def test_fp32c_families_relax_mode_and_step_cap():
