@@ -59,6 +59,11 @@ void make_react_dev(const Reaction &q, ReactDev &d) {
     d.lt2m1 = (float)(q.lt * q.lt - 1.0);
     d.lc_f = (float)q.lc;
     d.lc2m1 = (float)(q.lc * q.lc - 1.0);
+    double c = alpha;
+    for (int k = 1; k <= 7; ++k) {
+      c *= 0.5 * q.B / k;
+      d.pe[k - 1] = (float)c;
+    }
   }
 }
 

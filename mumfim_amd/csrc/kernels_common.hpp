@@ -199,6 +199,20 @@ __device__ inline float expm1_poly(float x) {
   return p * x;
 }
 
+// alpha expm1(B/2 y) for |B/2 y| <= kToePolyMax, as y P(y) with the
+// host-folded coefficients alpha (B/2)^k / k! (expm1_poly's Taylor series
+// with the argument and result scalings taken into the coefficients)
+__device__ inline float toe_poly(const ReactDev &q, float y) {
+  float p = q.pe[6];
+  p = p * y + q.pe[5];
+  p = p * y + q.pe[4];
+  p = p * y + q.pe[3];
+  p = p * y + q.pe[2];
+  p = p * y + q.pe[1];
+  p = p * y + q.pe[0];
+  return p * y;
+}
+
 // Branch-free: every lane evaluates both laws and both toe forms and selects.
 // expm1 on |x| <= kToePolyMax is the polynomial; beyond it exp(x) - 1 loses
 // at most a few ulp (e^x - 1 > 0.28 there).
@@ -225,8 +239,7 @@ __device__ inline float react32_tab(const float4 *T, unsigned rid, float e2, flo
 __device__ inline float react32_mix2(const ReactDev &nl, float ea_half_lin, bool is_nl,
                                      float e2, float il) {
   const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
-  const float x = nl.B_half * fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1);
-  const float tt = nl.alpha * il * expm1_poly(x);
+  const float tt = il * toe_poly(nl, fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1));
   const float lam2 = 1.0f + e2;
   const float ilam = __frsqrt_rn(lam2);
   const float lam = lam2 * ilam;

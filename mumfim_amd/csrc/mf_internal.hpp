@@ -153,6 +153,10 @@ struct ReactDev {
   float Kt, bt;    // tension extension: f = Kt*(lam - lt) + bt
   float lc_f, lc2m1; // compression transition lambda_c, lambda_c^2 - 1
   float Kc, bc;    // compression extension: f = Kc*(lam - lc) + bc
+  // alpha expm1(B/2 y) = y P(y) in the toe range: P's coefficients
+  // alpha (B/2)^k / k!, k = 1..7 (the Taylor polynomial with both scalings
+  // folded in)
+  float pe[7];
 };
 
 }  // namespace mf

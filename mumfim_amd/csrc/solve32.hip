@@ -679,8 +679,7 @@ void solve32r(SolveArgs a) {
           const float il = -c;
           const float e2 = sq * (il * il);
           const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-          const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
-          const float tt = nlq.alpha * il * expm1_poly(x);
+          const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
           const float lam2 = 1.0f + e2;
           const float ilam = __frsqrt_rn(lam2);
           const float lam = lam2 * ilam;
@@ -1143,8 +1142,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
         const float il = -c;
         const float e2 = sq * (il * il);
         const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-        const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
-        const float tt = nlq.alpha * il * expm1_poly(x);
+        const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
         const float lam2 = 1.0f + e2;
         const float ilam = __frsqrt_rn(lam2);
         const float lam = lam2 * ilam;
@@ -1584,8 +1582,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
         const float il = -c;
         const float e2 = sq * (il * il);
         const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-        const float x = nlq.B_half * fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1);
-        const float tt = nlq.alpha * il * expm1_poly(x);
+        const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
         const float lam2 = 1.0f + e2;
         const float ilam = __frsqrt_rn(lam2);
         const float lam = lam2 * ilam;
