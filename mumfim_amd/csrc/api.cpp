@@ -346,11 +346,12 @@ int make_plan(mf_batch *b, int prec) {
       // 256 threads x 2 linear nodes per thread (three solves per CU) beats
       // one node per thread at 512+ threads: 9% on 500-node linear RVEs
       const bool reg256x2 = regs && v.block == 256 && v.npt == 2;
-      // toe-law RVEs needing 640+ threads at one node per thread run faster
-      // fiber-parallel (up to 3 fibers per thread): 8% at 600 and 700 nodes,
-      // 16% at 800, 7% at 1000 (at 500 nodes and 512 threads the register
-      // kernel is 1.45x faster)
-      const bool regbig_nl = regs && !lin && v.block >= 640 && v.npt == 1;
+      // toe-law RVEs that leave more than an eighth of a 1024-thread register
+      // solve idle run faster fiber-parallel (8% at 800 nodes); fuller ones
+      // (1000 nodes) and the 640- and 768-thread shapes stay 2% faster in
+      // registers
+      const bool regbig_nl = regs && !lin && v.block == 1024 && v.npt == 1 &&
+                             8L * (p.Nf + p.n_split) <= 7L * 1024;
       // global-state FP64 last: only where nothing else fits.  FP64 with each
       // fiber evaluated once before the node-centric FP64 kernels for RVEs
       // with toe-law fibers (11% on 1500-node RVEs), after them for linear
