@@ -462,6 +462,34 @@ def test_large_rve_global_state_kernel():
         assert np.array_equal(sd.numpy(), s_eager)
 
 
+def test_reference_network_del_4450_global_state():
+    """The reference's largest test network, del_4450seedL5_new_1 (12308
+    nodes, 38376 fibers; test/microscale/fiber_networks, read by the
+    reference's global-stiffness test), through the C ABI.  It spans
+    [-2.5, 2.5]^3, so no node lies on the faces of the [-0.5, 0.5]^3 RVE
+    (RVE.h:86-103) and every node is free: the stress is zero and the
+    relax loop moves all 12308 nodes, beyond every LDS kernel, on the FP64
+    global-state kernel.  300 steps of the reference loop (tolerance out of
+    reach) leave the state bitwise as in the oracle."""
+    say("case: del_4450seedL5_new_1, 12308 free nodes")
+    net = mf.load_network(os.path.join(GOLDEN, "del_4450", "del_4450seedL5_new_1.txt"))
+    assert net.n_nodes == 12308 and net.n_fibers == 38376 and len(net.boundary) == 0
+    F = np.array([[1.01, 0.002, 0.0], [0.0, 0.995, 0.001], [0.0, 0.0, 1.0]])
+    p = mf.default_params(tolerance=1e-30, max_steps=300)
+    g = mf.BatchedFiberRVEAnalysisExplicit([net], p)
+    _, plan = g.plan()
+    assert plan[0, 2] == 8192, plan
+    s_g, st = g.run(F, return_status=True)
+    o = Oracle([net], p)
+    s_o, st_o = o.run(F, return_status=True)
+    assert st[0] == st_o[0] == mf.MF_STATUS_STEP_CAP
+    assert np.all(s_g == 0) and np.all(s_o == 0)
+    xg, ug, _ = g.get_state()
+    xo, uo, _ = o.get_state()
+    say(f"  max|u| {np.abs(uo).max():.3e}, max|du| {np.abs(ug - uo).max():.3e}")
+    assert np.array_equal(ug, uo) and np.array_equal(xg, xo)
+
+
 def test_fp32c_families_relax_mode_and_step_cap():
     """Reference (relax) mode through each FP32c kernel family: node-centric
     (register lists), fiber-parallel (300 nodes) and fiber-streamed (1300

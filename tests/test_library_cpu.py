@@ -120,3 +120,13 @@ def test_planner_takes_any_rve_size():
             assert "fits no kernel variant" not in str(e), e
     with pytest.raises(mf.MfError, match="fits no kernel variant"):
         mf.BatchedFiberRVEAnalysisExplicit([net], mf.default_params(energy_diagnostics=1))
+
+
+def test_load_reference_network_del_4450():
+    """The reference's largest test network (12308 nodes, 38376 fibers, one
+    reaction from its .params file) spans [-2.5, 2.5]^3: no node on the
+    [-0.5, 0.5]^3 RVE faces, so the loader finds no boundary node."""
+    net = mf.load_network(os.path.join(GOLDEN, "del_4450", "del_4450seedL5_new_1.txt"))
+    assert (net.n_nodes, net.n_fibers, len(net.boundary)) == (12308, 38376, 0)
+    assert np.isclose(np.abs(net.coords).max(), 2.5)
+    assert len(net.reactions) == 1 and net.reactions[0].E == 1e8
