@@ -104,7 +104,7 @@ def rel_err(a, b):
     return np.abs(a - b).max(1) / np.abs(b).max(1)
 
 
-def bench_cfg3(args):
+def bench_mixed16k(args):
     """16384 RVEs, N log-uniform in [200, 4000], half the fibers on the
     nonlinear toe reaction, F = I + sym(U) with U ~ U(-0.1, 0.1), 10000 ramp
     steps, FP32c in one batch (all kernel-variant groups launched together);
@@ -122,7 +122,7 @@ def bench_cfg3(args):
     g = mf.BatchedFiberRVEAnalysisExplicit(
         params=mf.default_params(precision=mf.MF_PREC_FP32C, **base),
         generated=(gen, R), n_nodes=sizes)
-    log(f"[cfg3] batch built in {time.perf_counter() - t:.1f}s, plan groups {g.plan()[0]}")
+    log(f"[mixed16k] batch built in {time.perf_counter() - t:.1f}s, plan groups {g.plan()[0]}")
     F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
     sig = mf.DeviceArray((R, 6))
     heur = None
@@ -132,7 +132,7 @@ def bench_cfg3(args):
         t = time.perf_counter()
         n_timed = g.autotune(mf.MF_PREC_FP32C, steps=args.tune_steps)
         tune_s = time.perf_counter() - t
-        log(f"[cfg3] autotuned in {tune_s:.1f}s ({n_timed} candidates timed), "
+        log(f"[mixed16k] autotuned in {tune_s:.1f}s ({n_timed} candidates timed), "
             f"plan groups {g.plan()[0]}, heuristic plan {heur}")
     wall, kms = time_solver(g, F_dev, sig, args.steps, args.warmup)
     s32 = sig.numpy()
@@ -145,7 +145,7 @@ def bench_cfg3(args):
         nets, mf.default_params(precision=mf.MF_PREC_FP64, **base))
     s64 = g64.run(F[sub], update_coords=False)
     e = rel_err(s32[sub], s64)
-    res = {"config": "cfg3", "workload": f"{R} RVEs, nodes log-uniform [200,4000] "
+    res = {"config": "mixed16k", "workload": f"{R} RVEs, nodes log-uniform [200,4000] "
            f"(total {int(st['N'].sum())} nodes, {int(st['M'].sum())} fibers), 50% nonlinear "
            f"fibers, F=I+sym(U) U~U(-0.1,0.1), {S} ramp steps",
            "value": round(R * args.steps / wall, 3), "unit": "RVE solves/s", "dtype": "f32",
@@ -163,7 +163,7 @@ def bench_cfg3(args):
     print(json.dumps(res), flush=True)
 
 
-def bench_cfg4(args):
+def bench_tangent1k2k(args):
     """4096 RVEs of 1000-2000 nodes, mixed reactions; per RVE the base solve
     plus 12 central perturbations (h = 1e-6) in one batched FP64 launch
     (53248 solves), FD tangent in FP64; tangent error vs the oracle on a
@@ -192,7 +192,7 @@ def bench_cfg4(args):
             times.append(dt)
     sigma, Cm = first
     wall = float(np.mean(times))
-    res = {"config": "cfg4", "workload": f"{R} RVEs, nodes U[1000,2000], 50% nonlinear fibers, "
+    res = {"config": "tangent1k2k", "workload": f"{R} RVEs, nodes U[1000,2000], 50% nonlinear fibers, "
            f"base + 12 central perturbations (h=1e-6) per RVE = {13 * R} solves of {S} ramp "
            f"steps, FP64 solves + FP64 differencing",
            "value": round(13 * R / wall, 3), "unit": "RVE solves/s (incl. perturbed)",
@@ -231,7 +231,7 @@ def bench_cfg4(args):
                             "sigma_err_vs_fp64_max": float(rel_err(s32, s_ref).max()),
                             "tangent_err_vs_fp64_max": float(eC.max()),
                             "tangent_err_vs_fp64_median": float(np.median(eC))}
-        log(f"[cfg4] fp32c solves, h={h:g}: {fp32[f'h={h:g}']}")
+        log(f"[tangent1k2k] fp32c solves, h={h:g}: {fp32[f'h={h:g}']}")
     res["fp32c_solves"] = fp32
     if not args.no_cpu:
         n = 16
@@ -246,9 +246,9 @@ def bench_cfg4(args):
     print(json.dumps(res), flush=True)
 
 
-def bench_cfg5(args):
-    """Learned surrogate.  Training data: explicit FP32c solves of config 2's
-    generator (500 nodes, k=4, linear fibers, 5000 ramp steps) with
+def bench_surrogate(args):
+    """Learned surrogate.  Training data: explicit FP32c solves of generated
+    networks (500 nodes, k=4, linear fibers, 5000 ramp steps) with
     F = I + sym(U), U ~ U(-0.1, 0.1): 65536 networks x 16 strains = 1,048,576
     solves.  An ensemble of 4 MLPs is trained on the GPU (BF16 MFMA, Adam).
     Inference is timed on 2^20 points (stress, and stress + tangent).  A
@@ -265,7 +265,7 @@ def bench_cfg5(args):
     t = time.perf_counter()
     g = mf.BatchedFiberRVEAnalysisExplicit(params=p32, generated=(mf.default_gen_params(seed=1),
                                                                  n_net))
-    log(f"[cfg5] {n_net} networks built in {time.perf_counter() - t:.1f}s")
+    log(f"[surrogate] {n_net} networks built in {time.perf_counter() - t:.1f}s")
     X = np.empty((n_data, mf.MF_MLP_IN), np.float32)
     Y = np.empty((n_data, 6), np.float32)
     F_dev = mf.DeviceArray((n_net, 9))
@@ -280,7 +280,7 @@ def bench_cfg5(args):
         solve_s += time.perf_counter() - t
         Y[k * n_net:(k + 1) * n_net] = sig_dev.numpy()
         X[k * n_net:(k + 1) * n_net] = mf.surrogate_inputs(g, F)
-        log(f"[cfg5] data {k + 1}/{per}: {n_net * (k + 1)} solves, {solve_s:.1f}s solving")
+        log(f"[surrogate] data {k + 1}/{per}: {n_net * (k + 1)} solves, {solve_s:.1f}s solving")
     explicit_rate = n_data / solve_s
     g.close()
     del F_dev, sig_dev
@@ -294,7 +294,7 @@ def bench_cfg5(args):
     t = time.perf_counter()
     hist = m.fit(Xd, Yd, n_data, **tp)
     train_s = time.perf_counter() - t
-    log(f"[cfg5] trained {tp['epochs']} epochs in {train_s:.1f}s, loss "
+    log(f"[surrogate] trained {tp['epochs']} epochs in {train_s:.1f}s, loss "
         f"{hist[0].mean():.4f} -> {hist[-1].mean():.4f}")
     # gradient of one member vs the CPU oracle on 512 training rows
     nm = m.normalization()
@@ -362,8 +362,8 @@ def bench_cfg5(args):
                      "path_ms_16384": round(stt["total_ms"], 3),
                      "points_per_s_1M": round(npts / t_1m, 1),
                      "speedup_vs_explicit": round(npts / t_1m / explicit_rate, 2)})
-        log(f"[cfg5] tol {tau:g}: {rows[-1]}")
-    res = {"config": "cfg5",
+        log(f"[surrogate] tol {tau:g}: {rows[-1]}")
+    res = {"config": "surrogate",
            "workload": f"ensemble of 4 MLPs 10-256-256-256-6 (SiLU, BF16 MFMA) trained on "
                        f"{n_data} FP32c solves ({n_net} networks of 500 nodes x {per} strains, "
                        f"U~U(-0.1,0.1), {S} ramp steps); inference on {npts} points; "
@@ -405,31 +405,31 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
-    ap.add_argument("--config", choices=["cfg2", "cfg3", "cfg4", "cfg5"], default="cfg2",
-                    help="cfg2 (default): 4096 x 500 nodes; cfg3: 16384 RVEs of 200-4000 "
-                         "nodes, mixed reactions, 10000 steps; cfg4: 4096 RVEs of "
-                         "1000-2000 nodes, base + 12 perturbed solves per RVE; cfg5: learned "
+    ap.add_argument("--config", choices=["batch500", "mixed16k", "tangent1k2k", "surrogate"], default="batch500",
+                    help="batch500 (default): 4096 x 500 nodes; mixed16k: 16384 RVEs of 200-4000 "
+                         "nodes, mixed reactions, 10000 steps; tangent1k2k: 4096 RVEs of "
+                         "1000-2000 nodes, base + 12 perturbed solves per RVE; surrogate: learned "
                          "surrogate trained on 1M solves, inference on 1M points")
     ap.add_argument("--surrogate-networks", type=int, default=65536,
-                    help="cfg5: networks in the training data (16 strains each)")
+                    help="surrogate: networks in the training data (16 strains each)")
     ap.add_argument("--surrogate-epochs", type=int, default=30)
-    ap.add_argument("--no-tune", action="store_true", help="cfg3: keep the heuristic plan")
+    ap.add_argument("--no-tune", action="store_true", help="mixed16k: keep the heuristic plan")
     ap.add_argument("--tune-steps", type=int, default=100,
-                    help="cfg3: ramp steps per autotuner candidate")
+                    help="mixed16k: ramp steps per autotuner candidate")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+        print(f"bench.py: mumfim_amd runs on one GPU; --gpus {args.gpus} is not "
               "supported (a multi-GPU run would shard the batch by RVE)", file=sys.stderr)
         sys.exit(2)
 
     import mumfim_amd as mf
 
-    if args.config == "cfg3":
-        return bench_cfg3(args)
-    if args.config == "cfg4":
-        return bench_cfg4(args)
-    if args.config == "cfg5":
-        return bench_cfg5(args)
+    if args.config == "mixed16k":
+        return bench_mixed16k(args)
+    if args.config == "tangent1k2k":
+        return bench_tangent1k2k(args)
+    if args.config == "surrogate":
+        return bench_surrogate(args)
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])

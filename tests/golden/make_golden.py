@@ -1,5 +1,6 @@
 """Extracts the numeric expectations the reference holds for the batched
-path into tests/golden/golden.json.  Run here (not on the GPU box):
+path into tests/golden/golden.json.  Run on a machine that holds the
+reference checkout (the tests read only the JSON it writes):
 
     python tests/golden/make_golden.py /root/reference
 

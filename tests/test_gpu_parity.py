@@ -158,9 +158,9 @@ def test_relax_multi_rve_broadcast_trial(net27):
 
 
 # ---------------------------------------------------------------- ramp mode
-def test_cfg1_ramp_fp64_and_fp32c():
+def test_ramp_500_nodes_5000_steps_fp64_and_fp32c():
     """Config 1: one RVE, 500 nodes, F = diag(1.05,1,1) ramped over 5000 steps."""
-    say("case: cfg1 ramp 500 nodes 5000 steps")
+    say("case: one 500-node RVE, ramp, 5000 steps")
     net = mf.generate_network(seed=1, n_nodes=500)
     F = np.diag([1.05, 1.0, 1.0])
     s_o = Oracle([net], ramp_params()).run(F)
@@ -176,10 +176,10 @@ def test_cfg1_ramp_fp64_and_fp32c():
     assert e32 <= FP32C_TOL
 
 
-def test_cfg2_subset_fp64_and_fp32c():
+def test_batch_500_nodes_subset_fp64_and_fp32c():
     """Config 2 generator and F distribution on 32 RVEs (seeds 1..32)."""
     R = 32
-    say(f"case: cfg2 subset R={R}")
+    say(f"case: 500-node batch subset R={R}")
     nets = [mf.generate_network(seed=1 + r, n_nodes=500) for r in range(R)]
     F = strain_F(2, R, 0.05)
     s_o = Oracle(nets, ramp_params()).run(F)
