@@ -23,8 +23,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RVE solves/s at ≤1e-4 rel stress err vs FP64; fraction of BW roofline"
 WORKLOAD = dict(n_rve=4096, n_nodes=500, k=4, explicit_steps=5000, F_amp=0.05)
-# MI355X LDS read bandwidth from the data sheet: 256 CUs x 128 B/clk x 2.4 GHz
-LDS_SPEC = 256 * 128 * 2.4e9
+# MI355X LDS read bandwidth: 256 CUs x 256 B/clk (gfx950 doubled the LDS read
+# width of gfx942's 128 B/clk; the measured ds_read_b128 rate, ~172 B/clk per
+# CU at 2.4 GHz, is above what 128 B/clk allows) x 2.4 GHz
+LDS_SPEC = 256 * 256 * 2.4e9
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # bench (scaled by a plain copy kernel of known size), see profiles/
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
