@@ -130,3 +130,15 @@ def test_load_reference_network_del_4450():
     assert (net.n_nodes, net.n_fibers, len(net.boundary)) == (12308, 38376, 0)
     assert np.isclose(np.abs(net.coords).max(), 2.5)
     assert len(net.reactions) == 1 and net.reactions[0].E == 1e8
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus above 1 exits with status 2 and says
+    so, before it loads the library or touches a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
