@@ -199,6 +199,36 @@ def test_batch_500_nodes_subset_fp64_and_fp32c():
     assert np.array_equal(g2.run(F), s_g)
 
 
+def test_full_size_batch_4096x500_fp32c_and_fp64():
+    """The full 4096 x 500-node, 5000-step batch: every FP32c stress within
+    1e-4 of the FP64 GPU stress, FP64 equal to the oracle on RVEs sampled
+    across the batch, every status OK, and a second FP32c batch built
+    the same way giving bitwise the same stress: the kernels have no
+    atomics or order-dependent sums."""
+    R = 4096
+    say(f"case: full 500-node batch R={R}, 5000 steps")
+    gen = mf.default_gen_params(seed=1, n_nodes=500)
+    F = strain_F(2, R, 0.05)
+    g64 = mf.BatchedFiberRVEAnalysisExplicit(params=ramp_params(), generated=(gen, R))
+    s64, st64 = g64.run(F, return_status=True)
+    g32 = mf.BatchedFiberRVEAnalysisExplicit(
+        params=ramp_params(precision=mf.MF_PREC_FP32C), generated=(gen, R))
+    s32, st32 = g32.run(F, return_status=True)
+    assert (st64 == 0).all() and (st32 == 0).all()
+    assert np.isfinite(s32).all()
+    e32 = rel_err(s32, s64)
+    sample = [0, 1, 1365, 2730, 4094, 4095]
+    nets = [mf.generate_network(seed=1 + r, n_nodes=500) for r in sample]
+    s_o = Oracle(nets, ramp_params()).run(F[sample])
+    e64 = rel_err(s64[sample], s_o)
+    say(f"  fp32c vs fp64 max {e32:.3e}; fp64 vs oracle on {len(sample)} RVEs {e64:.3e}")
+    assert e32 <= FP32C_TOL
+    assert e64 <= FP64_TOL
+    g32b = mf.BatchedFiberRVEAnalysisExplicit(
+        params=ramp_params(precision=mf.MF_PREC_FP32C), generated=(gen, R))
+    assert np.array_equal(g32b.run(F), s32)
+
+
 def test_uneven_mixed_reactions():
     """Config 3 shape on a few RVEs: log-uniform sizes, half nonlinear fibers."""
     sizes = [200, 430, 910, 1650, 2700, 4000]
