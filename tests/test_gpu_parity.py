@@ -229,6 +229,42 @@ def test_full_size_batch_4096x500_fp32c_and_fp64():
     assert np.array_equal(g32b.run(F), s32)
 
 
+def test_full_size_mixed_batch_16384_fp32c():
+    """16384 RVEs of 200-4000 nodes (log-uniform), half the fibers on the
+    toe law, F = I + sym(U) with U ~ U(-0.1, 0.1), 10000 ramp steps: every
+    RVE of the one FP32c batch (all kernel-variant groups launched together)
+    finishes with status OK and finite stress; 128 RVEs spread over the batch
+    agree with the FP64 GPU solver to 1e-4, and the two smallest of them
+    agree in FP64 with the oracle to 1e-10."""
+    R, S = 16384, 10000
+    say(f"case: full mixed batch R={R}, 200-4000 nodes, {S} steps")
+    u = mf.uniform(3, 11, R)
+    sizes = np.rint(np.exp(np.log(200) + u * (np.log(4000) - np.log(200)))).astype(np.int32)
+    F = strain_F(3, R, 0.1)
+    gen = mf.default_gen_params(seed=100001, nonlinear_fraction=0.5)
+    base = dict(n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER)
+    g32 = mf.BatchedFiberRVEAnalysisExplicit(
+        params=ramp_params(precision=mf.MF_PREC_FP32C, **base),
+        generated=(gen, R), n_nodes=sizes)
+    s32, st32 = g32.run(F, return_status=True)
+    assert (st32 == 0).all()
+    assert np.isfinite(s32).all()
+    sub = np.arange(0, R, R // 128)[:128]
+    nets = [mf.generate_network(seed=100001 + int(r), n_nodes=int(sizes[r]),
+                                nonlinear_fraction=0.5) for r in sub]
+    g64 = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(**base))
+    s64, st64 = g64.run(F[sub], return_status=True)
+    assert (st64 == 0).all()
+    e32 = rel_err(s32[sub], s64)
+    small = np.argsort(sizes[sub])[:2]
+    s_o = Oracle([nets[i] for i in small], ramp_params(**base)).run(F[sub][small])
+    e64 = rel_err(s64[small], s_o)
+    say(f"  fp32c vs fp64 on {len(sub)} RVEs {e32:.3e}; fp64 vs oracle "
+        f"({sizes[sub][small].tolist()} nodes) {e64:.3e}")
+    assert e32 <= FP32C_TOL
+    assert e64 <= FP64_TOL
+
+
 def test_uneven_mixed_reactions():
     """Config 3 shape on a few RVEs: log-uniform sizes, half nonlinear fibers."""
     sizes = [200, 430, 910, 1650, 2700, 4000]
