@@ -324,6 +324,33 @@ def test_ramp_tangent_central_fp64():
     assert np.array_equal(g2.computeMaterialStiffness(), C_g)
 
 
+def test_full_size_central_tangent_4096():
+    """4096 RVEs of 1000-2000 nodes, half the fibers on the toe law, base
+    solve + 12 central perturbations (h = 1e-6) of 5000 ramp steps each in
+    one FP64 launch: every stress and tangent entry finite, and the smallest
+    RVE's stress and tangent equal to the oracle's (1e-10, 1e-6)."""
+    R, S = 4096, 5000
+    say(f"case: full central tangent R={R}, 1000-2000 nodes, {S} steps, 13 solves each")
+    u = mf.uniform(4, 11, R)
+    sizes = (1000 + np.floor(u * 1001)).astype(np.int32)
+    F = strain_F(4, R, 0.05)
+    gen = mf.default_gen_params(seed=200001, nonlinear_fraction=0.5)
+    p = ramp_params(n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER,
+                    tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6)
+    g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R), n_nodes=sizes)
+    s_g, C_g = g.run_with_tangent(F)
+    assert np.isfinite(s_g).all() and np.isfinite(C_g).all()
+    r0 = int(np.argmin(sizes))
+    net = mf.generate_network(seed=200001 + r0, n_nodes=int(sizes[r0]), nonlinear_fraction=0.5)
+    o = Oracle([net], p)
+    s_o = o.run(F[r0:r0 + 1])
+    C_o = o.computeMaterialStiffness()
+    es, ec = rel_err(s_g[r0:r0 + 1], s_o), rel_err(C_g[r0:r0 + 1], C_o)
+    say(f"  RVE {r0} ({sizes[r0]} nodes): sigma {es:.3e}  C {ec:.3e}")
+    assert es <= FP64_TOL
+    assert ec <= 1e-6
+
+
 def test_ramp_trial_restores_state():
     say("case: ramp trial run leaves state unchanged")
     net = mf.generate_network(seed=9, n_nodes=400)
