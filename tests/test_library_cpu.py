@@ -142,3 +142,29 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/mumfim_amd.h compiles as C99 with -Wall -Werror (no C++
+    types cross the boundary), and a C program that calls entry points
+    links against libmumfim_amd.so. Nothing runs: no device is touched."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "use.c"
+    src.write_text(
+        '#include "mumfim_amd.h"\n'
+        "int main(void) {\n"
+        "  mf_solver_params p;\n"
+        "  mf_default_params(&p);\n"
+        "  mf_batch *b = 0;\n"
+        "  int rc = mf_batch_create(0, 0, &p, &b);\n"
+        "  return rc == MF_OK ? 1 : (mf_last_error() != 0 ? 0 : 2);\n"
+        "}\n")
+    lib = os.path.join(ROOT, "mumfim_amd")
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        str(src), "-o", str(tmp_path / "use"), "-L", lib, "-lmumfim_amd",
+                        f"-Wl,-rpath,{lib}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
