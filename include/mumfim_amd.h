@@ -9,13 +9,13 @@
  *   BatchedFiberRVEAnalysisExplicit(networks, strategies)   mf_batch_create
  *     (src/mumfim/microscale/BatchedFiberRVEAnalysisExplicit.h:80-256)
  *   BatchedRVEAnalysis::run(F, sigma, update_coords)        mf_batch_run / mf_batch_run_device
- *     (BatchedRVEAnalysis.h:48-50, BatchedFiberRVEAnalysisExplicit.h:257-320)
+ *     (BatchedRVEAnalysis.h:29-31, BatchedFiberRVEAnalysisExplicit.h:257-320)
  *   BatchedRVEAnalysis::computeMaterialStiffness(C)         mf_batch_tangent
- *     (BatchedRVEAnalysis.h:55-56, BatchedFiberRVEAnalysisExplicit.h:321-405)
+ *     (BatchedRVEAnalysis.h:36-37, BatchedFiberRVEAnalysisExplicit.h:321-405)
  *   compute3DOrientationTensor(omega)                       mf_batch_orientation_3d
- *     (BatchedRVEAnalysis.h:57-62, batched_analysis/BatchedComputeOrientationTensor.h:130-190)
+ *     (BatchedRVEAnalysis.h:38-43, batched_analysis/BatchedComputeOrientationTensor.h:130-190)
  *   compute2DOrientationTensor(normal, omega)               mf_batch_orientation_2d
- *     (BatchedRVEAnalysis.h:63-69, BatchedComputeOrientationTensor.h:56-128,191-208)
+ *     (BatchedRVEAnalysis.h:44-50, BatchedComputeOrientationTensor.h:56-128,191-208)
  *   std::exit(EXIT_FAILURE) on size mismatch                 negative return code + mf_last_error
  *     (BatchedFiberRVEAnalysisExplicit.h:85-90,324-330)
  *
@@ -54,7 +54,7 @@ extern "C" {
 #define MF_STATUS_FALLBACK 3     /* surrogate rejected, explicit solve used  */
 #define MF_STATUS_ENERGY 4       /* energy balance check failed (diagnostics) */
 
-/* ---- fiber reactions (FiberReactions.h:85-123, FiberNetworkIO_impl.h:123-145) */
+/* ---- fiber reactions (FiberReactions.h:15-43, FiberNetworkIO_impl.h:17-39) */
 #define MF_REACTION_LINEAR 0
 #define MF_REACTION_NONLINEAR 1
 
@@ -283,7 +283,7 @@ int mf_batch_create_generated(const mf_gen_params *g, int32_t n_rve,
 /* Uniform doubles in [0,1) from the SplitMix64 stream (seed, stream). */
 int mf_uniform(uint64_t seed, uint64_t stream, int64_t n, double *out);
 
-/* ---- reference text formats (FiberNetworkIO.cc:47-106, FiberNetworkIO_impl.h:17-193) */
+/* ---- reference text formats (FiberNetworkIO.cc:47-106, FiberNetworkIO_impl.h:17-39,68-87) */
 /* Reads "<nn> <ne> <np>\n x y z ...\n n0 n1 ..." and the optional
  * "<path>.params" file.  Boundary nodes are those within 1e-8 of a face of
  * [-0.5, 0.5]^3 (RVE.h:86-103).  Two-call protocol as mf_generate_network;

@@ -194,7 +194,8 @@ struct mf_batch {
     double *sigma;
     int32_t *status;
     int cur, prec, version;
-    const void *scratch[5];  // library buffers the captured launches use
+    static constexpr int kScratch = 8;
+    const void *scratch[kScratch];  // library buffers the captured launches use
     hipGraphExec_t exec;
   };
   bool use_graph = false;
@@ -895,13 +896,16 @@ int run_device_graph(mf_batch *b, const double *F_dev, int nF, int update, doubl
   MF_HIP(b->status.ensure(R));
   if (prec == MF_PREC_FP64 && b->p.energy_diagnostics) MF_HIP(b->energy.ensure(4 * (size_t)R));
   if (update && ramp) MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));
-  const void *scratch[5] = {b->sigma.p, b->status.p, b->energy.p, b->lastF_dev.p,
-                            b->eta_rve.p};
+  // every library buffer the captured launches read or write: a buffer that
+  // was reallocated since the capture makes the graph stale
+  const void *scratch[mf_batch::Graph::kScratch] = {
+      b->sigma.p, b->status.p, b->energy.p, b->lastF_dev.p,
+      b->eta_rve.p, b->steps.p, b->gstate.p, b->cur_stress.p};
   mf_batch::Graph *g = nullptr;
   for (auto &x : b->graphs)
     if (x.F == F_dev && x.nF == nF && x.update == update && x.sigma == sigma_dev &&
         x.status == status_dev && x.cur == b->cur && x.prec == prec &&
-        x.version == b->plan_version && std::equal(scratch, scratch + 5, x.scratch))
+        x.version == b->plan_version && std::equal(scratch, scratch + mf_batch::Graph::kScratch, x.scratch))
       g = &x;
   if (!g) {
     if (b->graphs.size() >= 8) b->drop_graphs();
@@ -932,7 +936,7 @@ int run_device_graph(mf_batch *b, const double *F_dev, int nF, int update, doubl
     MF_HIP(ie);
     mf_batch::Graph ng{F_dev, nF, update, sigma_dev, status_dev, b->cur, prec, b->plan_version,
                        {}, exec};
-    std::copy(scratch, scratch + 5, ng.scratch);
+    std::copy(scratch, scratch + mf_batch::Graph::kScratch, ng.scratch);
     b->graphs.push_back(ng);
     g = &b->graphs.back();
   }
@@ -979,38 +983,59 @@ int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
     set_error("computeMaterialStiffness needs a previous run with update_coords");
     return MF_ERR_STATE;
   }
+  // The forward scheme differences each perturbed solve against a base
+  // stress of the same precision.  When the perturbed solves run at another
+  // precision than run(), the committed stress cannot serve as that base
+  // (their difference would carry the precision gap divided by h), so an
+  // unperturbed solve at tangent_precision provides it.
+  const bool own_base = !central && b->p.tangent_precision != b->p.precision;
   DevBuf<double> sig_p;
   MF_HIP(sig_p.alloc(6 * (size_t)R));
   if (b->p.mode == MF_MODE_RELAX) {
-    DevBuf<double> sig_m;
+    DevBuf<double> sig_m, sig_b;
     if (central) MF_HIP(sig_m.alloc(6 * (size_t)R));
     std::vector<int32_t> st_all(R, MF_STATUS_OK), st(R);
+    // one trial: every RVE from the current coordinates (kept afterwards,
+    // as the reference's trials keep theirs), u restored
+    auto trial = [&](const double Fp[9], double *sig) -> int {
+      SolveSet S;
+      S.prec = b->p.tangent_precision;
+      S.rve.resize(R);
+      S.flags.assign(R, kWriteX);
+      S.F.resize(9 * (size_t)R);
+      for (int r = 0; r < R; ++r) {
+        S.rve[r] = r;
+        std::copy(Fp, Fp + 9, S.F.begin() + 9 * (size_t)r);
+      }
+      const int c = b->cur;
+      int e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c].p, b->us[c].p, sig);
+      if (e != MF_OK) return e;
+      MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * R,
+                            hipMemcpyDeviceToHost, s));
+      MF_HIP(hipStreamSynchronize(s));
+      for (int r = 0; r < R; ++r)
+        if (st_all[r] == MF_STATUS_OK) st_all[r] = st[r];
+      return MF_OK;
+    };
+    const double* base = b->cur_stress.p;
+    if (own_base) {
+      // zero-increment trial: the committed state relaxed at tangent_precision
+      MF_HIP(sig_b.alloc(6 * (size_t)R));
+      const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      int e = trial(I3, sig_b.p);
+      if (e != MF_OK) return e;
+      base = sig_b.p;
+    }
     for (int i = 0; i < 6; ++i) {
       for (int sign = 0; sign < (central ? 2 : 1); ++sign) {
         double Fp[9];
         perturbation(i, sign ? -h : h, Fp);
-        SolveSet S;
-        S.prec = b->p.tangent_precision;
-        S.rve.resize(R);
-        S.flags.assign(R, kWriteX);
-        S.F.resize(9 * (size_t)R);
-        for (int r = 0; r < R; ++r) {
-          S.rve[r] = r;
-          std::copy(Fp, Fp + 9, S.F.begin() + 9 * (size_t)r);
-        }
-        const int c = b->cur;
-        int e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c].p, b->us[c].p,
-                           sign ? sig_m.p : sig_p.p);
+        int e = trial(Fp, sign ? sig_m.p : sig_p.p);
         if (e != MF_OK) return e;
-        MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * R,
-                              hipMemcpyDeviceToHost, s));
-        MF_HIP(hipStreamSynchronize(s));
-        for (int r = 0; r < R; ++r)
-          if (st_all[r] == MF_STATUS_OK) st_all[r] = st[r];
       }
       hipError_t e = central
           ? launch_fd(sig_p.p, sig_m.p, Cdev, R, i, 2 * h, 6, 6, s)
-          : launch_fd(sig_p.p, b->cur_stress.p, Cdev, R, i, h, 6, 6, s);
+          : launch_fd(sig_p.p, base, Cdev, R, i, h, 6, 6, s);
       if (e != hipSuccess) {
         set_error(hipGetErrorString(e));
         return MF_ERR_HIP;
@@ -1037,6 +1062,11 @@ int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
   S.prec = b->p.tangent_precision;
   for (int r = 0; r < R; ++r) {
     const double *F0 = b->lastF.data() + 9 * (size_t)r;
+    if (own_base) {  // the update run's solve again, at tangent_precision
+      S.rve.push_back(r);
+      S.flags.push_back(0);
+      S.F.insert(S.F.end(), F0, F0 + 9);
+    }
     for (int k = 1; k < per; ++k) {
       int i = central ? (k - 1) / 2 : k - 1;
       double sgn = (central && ((k - 1) % 2 == 1)) ? -1.0 : 1.0;
@@ -1051,17 +1081,22 @@ int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
   const int p = b->cur ^ 1;  // state before the last update run
   int e = launch_set(b, S, b->xs[p].p, b->us[p].p, nullptr, nullptr);
   if (e != MF_OK) return e;
-  // assemble [base, perturbed...] per RVE for the FD kernel
+  // [base, perturbed...] per RVE for the FD kernel: the solve set's own
+  // layout when it holds the base, else the committed stress in front
   DevBuf<double> all;
-  MF_HIP(all.alloc((size_t)R * per * 6));
-  MF_HIP(hipMemcpy2DAsync(all.p, sizeof(double) * 6 * per, b->cur_stress.p,
-                          sizeof(double) * 6, sizeof(double) * 6, R,
-                          hipMemcpyDeviceToDevice, s));
-  MF_HIP(hipMemcpy2DAsync(all.p + 6, sizeof(double) * 6 * per, b->sigma.p,
-                          sizeof(double) * 6 * (per - 1),
-                          sizeof(double) * 6 * (per - 1), R,
-                          hipMemcpyDeviceToDevice, s));
-  hipError_t he = launch_fd_all(all.p, Cdev, R,
+  const double *rows = b->sigma.p;
+  if (!own_base) {
+    MF_HIP(all.alloc((size_t)R * per * 6));
+    MF_HIP(hipMemcpy2DAsync(all.p, sizeof(double) * 6 * per, b->cur_stress.p,
+                            sizeof(double) * 6, sizeof(double) * 6, R,
+                            hipMemcpyDeviceToDevice, s));
+    MF_HIP(hipMemcpy2DAsync(all.p + 6, sizeof(double) * 6 * per, b->sigma.p,
+                            sizeof(double) * 6 * (per - 1),
+                            sizeof(double) * 6 * (per - 1), R,
+                            hipMemcpyDeviceToDevice, s));
+    rows = all.p;
+  }
+  hipError_t he = launch_fd_all(rows, Cdev, R,
                                 central ? MF_TANGENT_CENTRAL : MF_TANGENT_FORWARD,
                                 h, s);
   if (he != hipSuccess) {
@@ -1529,6 +1564,7 @@ int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
   const int c = b->cur;
   e = launch_set(b, S, b->xs[c].p, b->us[c].p, b->xs[c ^ 1].p, b->us[c ^ 1].p);
   if (e != MF_OK) return e;
+  b->have_energy = false;  // this run's solves track no energies
   DevBuf<double> Cd;
   MF_HIP(Cd.alloc(36 * (size_t)R));
   he = launch_fd_all(b->sigma.p, Cd.p, R,

@@ -39,7 +39,7 @@ enum : uint64_t { kStreamNodes = 1, kStreamReactions = 2 };
 struct Reaction {
   int type = MF_REACTION_LINEAR;
   double radius = 0, E = 0, rho = 0, B = 0, lt = 0, lc = 1.0;
-  // A = r*r*pi, evaluated as in FiberNetworkIO_impl.h:132.
+  // A = r*r*pi, evaluated as in FiberNetworkIO_impl.h:26,36.
   double area() const { return radius * radius * 3.14159265358979323846; }
 };
 

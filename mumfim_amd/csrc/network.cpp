@@ -191,7 +191,7 @@ int generate_network(const mf_gen_params &g, Network &out) {
 // np periodic pairs (ignored by the explicit solver), FiberNetworkIO.cc:47-106.
 // Params file "<path>.params": "nr ne", nr reaction lines
 // ("0 r E rho" linear, "1 r E B lt rho" nonlinear with lc = 1),
-// then ne reaction ids, FiberNetworkIO_impl.h:123-193.
+// then ne reaction ids, FiberNetworkIO_impl.h:17-39,68-87.
 int load_network_file(const std::string &path, Network &out) {
   std::ifstream is(path);
   if (!is) {

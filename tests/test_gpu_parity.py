@@ -233,9 +233,11 @@ def test_full_size_mixed_batch_16384_fp32c():
     """16384 RVEs of 200-4000 nodes (log-uniform), half the fibers on the
     toe law, F = I + sym(U) with U ~ U(-0.1, 0.1), 10000 ramp steps: every
     RVE of the one FP32c batch (all kernel-variant groups launched together)
-    finishes with status OK and finite stress; 128 RVEs spread over the batch
-    agree with the FP64 GPU solver to 1e-4, and the two smallest of them
-    agree in FP64 with the oracle to 1e-10."""
+    finishes with status OK and finite stress.  A sample goes to the oracle
+    at the full 10000 steps: 128 RVEs spread over the batch, the 4 largest
+    RVEs, and the largest RVE of every FP32c kernel variant in the plan.  On
+    it the FP64 GPU solver matches the oracle to 1e-10 and the FP32c batch
+    matches the oracle (and the FP64 GPU solver) to 1e-4."""
     R, S = 16384, 10000
     say(f"case: full mixed batch R={R}, 200-4000 nodes, {S} steps")
     u = mf.uniform(3, 11, R)
@@ -249,20 +251,36 @@ def test_full_size_mixed_batch_16384_fp32c():
     s32, st32 = g32.run(F, return_status=True)
     assert (st32 == 0).all()
     assert np.isfinite(s32).all()
-    sub = np.arange(0, R, R // 128)[:128]
+    _, plan32 = g32.plan()
+    g32.close()
+    sub = set(int(r) for r in np.arange(0, R, R // 128)[:128])
+    sub |= set(int(r) for r in np.argsort(sizes, kind="stable")[-4:])
+    fam = {}
+    for r in range(R):
+        k = tuple(int(v) for v in plan32[r, :3])
+        if k not in fam or sizes[r] > sizes[fam[k]]:
+            fam[k] = r
+    sub |= set(fam.values())
+    sub = np.array(sorted(sub))
+    say(f"  oracle sample: {len(sub)} RVEs ({len(fam)} FP32c variants), "
+        f"{sizes[sub].min()}-{sizes[sub].max()} nodes")
     nets = [mf.generate_network(seed=100001 + int(r), n_nodes=int(sizes[r]),
                                 nonlinear_fraction=0.5) for r in sub]
     g64 = mf.BatchedFiberRVEAnalysisExplicit(nets, ramp_params(**base))
     s64, st64 = g64.run(F[sub], return_status=True)
     assert (st64 == 0).all()
+    s_o, st_o = Oracle(nets, ramp_params(**base)).run(F[sub], return_status=True)
+    assert (st_o == 0).all()
+    e64 = rel_err(s64, s_o)
+    e32o = rel_err(s32[sub], s_o)
     e32 = rel_err(s32[sub], s64)
-    small = np.argsort(sizes[sub])[:2]
-    s_o = Oracle([nets[i] for i in small], ramp_params(**base)).run(F[sub][small])
-    e64 = rel_err(s64[small], s_o)
-    say(f"  fp32c vs fp64 on {len(sub)} RVEs {e32:.3e}; fp64 vs oracle "
-        f"({sizes[sub][small].tolist()} nodes) {e64:.3e}")
-    assert e32 <= FP32C_TOL
+    big = np.argsort(sizes[sub], kind="stable")[-4:]
+    say(f"  fp64 vs oracle {e64:.3e}; fp32c vs oracle {e32o:.3e}; fp32c vs fp64 {e32:.3e}; "
+        f"4 largest ({sizes[sub][big].tolist()} nodes): fp64 {rel_err(s64[big], s_o[big]):.3e} "
+        f"fp32c {rel_err(s32[sub][big], s_o[big]):.3e}")
     assert e64 <= FP64_TOL
+    assert e32o <= FP32C_TOL
+    assert e32 <= FP32C_TOL
 
 
 def test_uneven_mixed_reactions():
@@ -300,20 +318,24 @@ def test_kernel_variants_agree_bitwise(block):
     assert np.array_equal(g.run(F), ref)
 
 
-def test_ramp_tangent_central_fp64():
-    """Config 4 shape: base + 12 central perturbations (h = 1e-6) in one launch."""
+@pytest.mark.parametrize("scheme", [mf.MF_TANGENT_FORWARD, mf.MF_TANGENT_CENTRAL])
+def test_ramp_tangent_fp64(scheme):
+    """Config 4 shape: base + 6 forward or 12 central perturbations
+    (h = 1e-6) in one launch (the fused run + tangent), then the same tangent
+    from a separate run and tangent call."""
     R = 4
-    say("case: ramp central tangent")
+    say(f"case: ramp tangent scheme={scheme}")
     nets = [mf.generate_network(seed=300 + r, n_nodes=n, nonlinear_fraction=0.5)
             for r, n in enumerate([300, 500, 800, 1000])]
     F = strain_F(5, R, 0.05)
     p = ramp_params(n_steps=1500, reaction_policy=mf.MF_REACT_PER_FIBER,
-                    tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6)
+                    tangent_scheme=scheme, fd_step=1e-6)
     o = Oracle(nets, p)
     s_o = o.run(F)
     C_o = o.computeMaterialStiffness()
     g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
-    s_g, C_g = g.run_with_tangent(F)
+    s_g, C_g, st = g.run_with_tangent(F, return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
     es, ec = rel_err(s_g, s_o), rel_err(C_g, C_o)
     say(f"  sigma {es:.3e}  C {ec:.3e}")
     assert es <= FP64_TOL
@@ -321,14 +343,138 @@ def test_ramp_tangent_central_fp64():
     # separate run + tangent gives the same numbers as the fused launch
     g2 = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
     g2.run(F)
-    assert np.array_equal(g2.computeMaterialStiffness(), C_g)
+    C2, st2 = g2.computeMaterialStiffness(return_status=True)
+    assert (st2 == mf.MF_STATUS_OK).all()
+    assert np.array_equal(C2, C_g)
+
+
+def _affine_patch():
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-0.5, 0.5, (12, 3))
+    fb = np.array([[i, j] for i in range(12) for j in range(i + 1, 12) if (i + j) % 3 == 0],
+                  np.int32)
+    return mf.Network(X, fb, np.arange(12, dtype=np.int32), [mf.Reaction(0, 0.01, 1e5, 1000.0)])
+
+
+def _perturbation(i, h):
+    """Almansi perturbation of Voigt direction i (BatchedFiberRVEAnalysisExplicit.h:338-390)."""
+    Fp = np.eye(3)
+    if i < 3:
+        Fp[i, i] = math.sqrt(1.0 / (1 - 2 * h))
+        return Fp
+    l2, l3 = (1 + h) / (1 - h * h), (1 - h) / (1 - h * h)
+    j, k = [(1, 2), (0, 2), (0, 1)][i - 3]
+    Fp[j, j] = Fp[k, k] = 0.5 * (math.sqrt(l2) + math.sqrt(l3))
+    Fp[j, k] = Fp[k, j] = 0.5 * (math.sqrt(l2) - math.sqrt(l3))
+    return Fp
+
+
+@pytest.mark.parametrize("scheme", [mf.MF_TANGENT_FORWARD, mf.MF_TANGENT_CENTRAL])
+def test_ramp_tangent_affine_patch_closed_form(scheme):
+    """The GPU ramp-mode tangent on the all-boundary patch after two update
+    runs F1, F2 equals the difference quotient of the closed-form affine
+    stresses at P_i(+-h) F2 F1 X over det F1 det F2, in both precisions."""
+    say(f"case: ramp tangent, all-boundary patch, scheme={scheme}")
+    net = _affine_patch()
+    X, fb = net.coords, net.fibers
+    r0 = net.reactions[0]
+    EA = r0.E * math.pi * r0.radius ** 2
+    F1 = np.array([[1.02, 0.01, 0.0], [0.0, 0.99, 0.005], [0.003, 0.0, 1.01]])
+    F2 = np.array([[0.995, 0.0, 0.012], [0.004, 1.015, 0.0], [0.0, -0.008, 1.0]])
+    V = np.linalg.det(F1) * np.linalg.det(F2)
+    h = 1e-6
+    central = scheme == mf.MF_TANGENT_CENTRAL
+
+    def sig(x):
+        d = x[fb[:, 1]] - x[fb[:, 0]]
+        l = np.linalg.norm(d, axis=1)
+        l0 = np.linalg.norm(X[fb[:, 1]] - X[fb[:, 0]], axis=1)
+        lam = l / l0
+        S = np.einsum("e,ei,ej->ij", lam * EA * 0.5 * (lam ** 2 - 1) / l, d, d) / V
+        return np.array([S[0, 0], S[1, 1], S[2, 2], S[1, 2], S[0, 2], S[0, 1]])
+
+    x1 = X @ F1.T
+    s0 = sig(x1 @ F2.T)
+    Cc = np.zeros((6, 6))
+    for i in range(6):
+        t = [sig(x1 @ (_perturbation(i, sg * h) @ F2).T) for sg in ((1, -1) if central else (1,))]
+        Cc[:, i] = (t[0] - t[1]) / (2 * h) if central else (t[0] - s0) / h
+    for prec in (mf.MF_PREC_FP64, mf.MF_PREC_FP32C):
+        p = ramp_params(n_steps=200, tangent_scheme=scheme, fd_step=h, precision=prec,
+                        tangent_precision=mf.MF_PREC_FP64)
+        g = mf.BatchedFiberRVEAnalysisExplicit([net], p)
+        g.run(F1)
+        s_c = g.run(F2)[0]
+        C, st = g.computeMaterialStiffness(return_status=True)
+        e_s = np.abs(s_c - s0).max() / np.abs(s0).max()
+        e_c = np.abs(C[0] - Cc).max() / np.abs(Cc).max()
+        say(f"  precision {prec}: sigma {e_s:.2e}  C {e_c:.2e}")
+        assert (st == 0).all()
+        assert e_s <= (1e-12 if prec == mf.MF_PREC_FP64 else 1e-6)
+        assert e_c <= 1e-7
+
+
+@pytest.mark.parametrize("mode", [mf.MF_MODE_RAMP, mf.MF_MODE_RELAX])
+def test_forward_tangent_mixed_precision(mode, golden, net27):
+    """run() in FP32c with the forward tangent's solves in FP64: the base of
+    the differences is an FP64 solve too, so C matches the all-FP64 tangent
+    instead of carrying (sigma_fp32c - sigma_fp64)/h.  Ramp mode: the same
+    FP64 solves from the same state, equal to rounding of the stress sums.
+    Relax mode (Network_2_7 under the golden F, reference settings): equal to
+    the relaxation tolerance the reference's own golden tangent blocks show
+    (5e-3)."""
+    say(f"case: forward tangent, FP32c run, FP64 tangent solves, mode={mode}")
+    if mode == mf.MF_MODE_RAMP:
+        nets = [mf.generate_network(seed=330 + r, n_nodes=n, E=1e8, radius=5e-5,
+                                    nonlinear_fraction=0.5) for r, n in enumerate([300, 600])]
+        F = strain_F(7, 2, 0.02)
+        kw = dict(mode=mode, n_steps=1500, reaction_policy=mf.MF_REACT_PER_FIBER, fd_step=1e-6)
+    else:
+        nets = [net27]
+        F = np.array(golden["network_2_7"]["F"])[None]
+        kw = dict(mode=mode, max_steps=200000)
+    kw.update(tangent_scheme=mf.MF_TANGENT_FORWARD, tangent_precision=mf.MF_PREC_FP64)
+    g64 = mf.BatchedFiberRVEAnalysisExplicit(nets, mf.default_params(**kw))
+    s64 = g64.run(F)
+    C64 = g64.computeMaterialStiffness()
+    gm = mf.BatchedFiberRVEAnalysisExplicit(
+        nets, mf.default_params(precision=mf.MF_PREC_FP32C, **kw))
+    s32 = gm.run(F)
+    Cm, st = gm.computeMaterialStiffness(return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
+    e = rel_err(Cm, C64)
+    say(f"  sigma fp32c vs fp64 {rel_err(s32, s64):.2e}; C mixed vs fp64 {e:.2e}")
+    assert e <= (1e-9 if mode == mf.MF_MODE_RAMP else 5e-3)
+    # the fused entry point takes the same route
+    gf = mf.BatchedFiberRVEAnalysisExplicit(
+        nets, mf.default_params(precision=mf.MF_PREC_FP32C, **kw))
+    _, Cf = gf.run_with_tangent(F)
+    assert rel_err(Cf, C64) <= (1e-9 if mode == mf.MF_MODE_RAMP else 5e-3)
+
+
+def test_energies_after_fused_run_with_tangent():
+    """The fused ramp run + tangent tracks no energies, so the energies of
+    an earlier run are not reported as its own."""
+    say("case: energies after run_with_tangent")
+    nets = [mf.generate_network(seed=340, n_nodes=300)]
+    p = ramp_params(n_steps=300, energy_diagnostics=1, tangent_scheme=mf.MF_TANGENT_CENTRAL)
+    g = mf.BatchedFiberRVEAnalysisExplicit(nets, p)
+    g.run(np.diag([1.01, 1.0, 1.0]))
+    assert np.isfinite(g.last_energies()).all()
+    g.run_with_tangent(np.diag([1.005, 1.0, 1.0])[None])
+    with pytest.raises(mf.MfError):
+        g.last_energies()
 
 
 def test_full_size_central_tangent_4096():
     """4096 RVEs of 1000-2000 nodes, half the fibers on the toe law, base
     solve + 12 central perturbations (h = 1e-6) of 5000 ramp steps each in
-    one FP64 launch: every stress and tangent entry finite, and the smallest
-    RVE's stress and tangent equal to the oracle's (1e-10, 1e-6)."""
+    one FP64 launch: every status OK, every stress and tangent entry finite.
+    A second tangent at h = 1e-5 from the same state ranks the RVEs by how
+    much their tangent moves with h (fibers near a toe-law transition).
+    Against the oracle (full 5000 steps, h = 1e-6): the smallest and largest
+    RVEs, the 8 most h-sensitive ones and 14 spread over the batch, stress
+    to 1e-10 and tangent to 1e-6."""
     R, S = 4096, 5000
     say(f"case: full central tangent R={R}, 1000-2000 nodes, {S} steps, 13 solves each")
     u = mf.uniform(4, 11, R)
@@ -338,15 +484,28 @@ def test_full_size_central_tangent_4096():
     p = ramp_params(n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER,
                     tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6)
     g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R), n_nodes=sizes)
-    s_g, C_g = g.run_with_tangent(F)
+    s_g, C_g, st = g.run_with_tangent(F, return_status=True)
+    assert (st == mf.MF_STATUS_OK).all()
     assert np.isfinite(s_g).all() and np.isfinite(C_g).all()
-    r0 = int(np.argmin(sizes))
-    net = mf.generate_network(seed=200001 + r0, n_nodes=int(sizes[r0]), nonlinear_fraction=0.5)
-    o = Oracle([net], p)
-    s_o = o.run(F[r0:r0 + 1])
+    g.set_params(fd_step=1e-5)
+    C5, st5 = g.computeMaterialStiffness(return_status=True)
+    assert (st5 == mf.MF_STATUS_OK).all()
+    g.close()
+    sens = np.abs(C_g - C5).reshape(R, -1).max(1) / np.abs(C5).reshape(R, -1).max(1)
+    pick = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    pick |= set(int(r) for r in np.argsort(sens, kind="stable")[-8:])
+    pick |= set(int(r) for r in np.arange(0, R, R // 14)[:14])
+    pick = np.array(sorted(pick))
+    nets = [mf.generate_network(seed=200001 + int(r), n_nodes=int(sizes[r]),
+                                nonlinear_fraction=0.5) for r in pick]
+    o = Oracle(nets, p)
+    s_o = o.run(F[pick])
     C_o = o.computeMaterialStiffness()
-    es, ec = rel_err(s_g[r0:r0 + 1], s_o), rel_err(C_g[r0:r0 + 1], C_o)
-    say(f"  RVE {r0} ({sizes[r0]} nodes): sigma {es:.3e}  C {ec:.3e}")
+    es, ec = rel_err(s_g[pick], s_o), rel_err(C_g[pick], C_o)
+    top = np.argsort(sens[pick], kind="stable")[-8:]
+    say(f"  {len(pick)} RVEs ({sizes[pick].min()}-{sizes[pick].max()} nodes) vs oracle: "
+        f"sigma {es:.3e}  C {ec:.3e}; 8 most h-sensitive (|C(1e-6)-C(1e-5)| up to "
+        f"{sens.max():.2e}): C {rel_err(C_g[pick][top], C_o[top]):.3e}")
     assert es <= FP64_TOL
     assert ec <= 1e-6
 
@@ -374,6 +533,9 @@ def test_orientation_tensors_match_oracle(golden, net27):
     assert rel_err(g.compute3DOrientationTensor(), o.orientation()) <= 1e-13
     n = [0.3, 0.2, 0.9]
     assert rel_err(g.compute2DOrientationTensor(n), o.orientation(n)) <= 1e-13
+    # non-unit normals: the reference divides by |n|^2, a quirk both keep
+    for n in ([0.0, 0.0, 2.0], [0.0, 0.5, 0.0], [1.5, -0.7, 0.4]):
+        assert rel_err(g.compute2DOrientationTensor(n), o.orientation(n)) <= 1e-13
 
 
 def test_edge_cases_all_boundary_and_no_boundary():

@@ -354,3 +354,86 @@ def test_energy_check_status(net27):
                               energy_check_eps=eps)
         _, st = Oracle([net27], p).run(F, return_status=True)
         assert st[0] == want
+
+
+def _affine_patch():
+    """12 nodes, every one prescribed, linear fibers: the stress of any
+    state the boundary reaches is the fiber virial of that affine state."""
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-0.5, 0.5, (12, 3))
+    fb = np.array([[i, j] for i in range(12) for j in range(i + 1, 12) if (i + j) % 3 == 0],
+                  np.int32)
+    return mf.Network(X, fb, np.arange(12, dtype=np.int32),
+                      [mf.Reaction(0, 0.01, 1e5, 1000.0)])
+
+
+@pytest.mark.parametrize("scheme", [mf.MF_TANGENT_FORWARD, mf.MF_TANGENT_CENTRAL])
+def test_ramp_tangent_affine_patch_closed_form(scheme):
+    """Ramp-mode tangent (no reference counterpart) on the all-boundary
+    patch after two update runs F1 then F2.  A ramp solve moves every node
+    to F x_start, so the perturbed solve i(+-h) ends at P_i(+-h) F2 F1 X and
+    its stress is the closed-form virial of that state over the committed
+    volume det F1 det F2.  The tangent must be the difference quotient of
+    those closed forms: P_i applied after F2 (the perturbation is a further
+    deformation of the current configuration,
+    BatchedFiberRVEAnalysisExplicit.h:331-403), from the state before the
+    last update run (F1 X).  Composing F2 P_i instead, or starting from X or
+    from the committed state, moves C at O(|F - I|), which the test shows."""
+    net = _affine_patch()
+    X = net.coords
+    F1 = np.array([[1.02, 0.01, 0.0], [0.0, 0.99, 0.005], [0.003, 0.0, 1.01]])
+    F2 = np.array([[0.995, 0.0, 0.012], [0.004, 1.015, 0.0], [0.0, -0.008, 1.0]])
+    h = 1e-6
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=200, tangent_scheme=scheme, fd_step=h)
+    o = Oracle([net], p)
+    o.run(F1)
+    s_c = o.run(F2)[0]
+    V = np.linalg.det(F1) * np.linalg.det(F2)
+
+    def sig(x):
+        return _fiber_virial(net, x) / V
+
+    x1 = X @ F1.T
+    s0 = sig(x1 @ F2.T)
+    assert np.allclose(s_c, s0, rtol=0, atol=1e-12 * np.abs(s0).max())
+    C = o.computeMaterialStiffness()[0]
+    central = scheme == mf.MF_TANGENT_CENTRAL
+
+    def closed(order, start):
+        Cx = np.zeros((6, 6))
+        for i in range(6):
+            t = []
+            for sg in ((1, -1) if central else (1,)):
+                Pq = _perturbation(i, sg * h)
+                Ft = Pq @ F2 if order == "PF" else F2 @ Pq
+                t.append(sig(start @ Ft.T))
+            Cx[:, i] = (t[0] - t[1]) / (2 * h) if central else (t[0] - s0) / h
+        return Cx
+
+    Cc = closed("PF", x1)
+    scale = np.abs(Cc).max()
+    assert np.abs(C - Cc).max() <= 1e-7 * scale, np.abs(C - Cc).max() / scale
+    # the plausible mistakes are visible
+    for wrong in (closed("FP", x1), closed("PF", X), closed("PF", x1 @ F2.T)):
+        assert np.abs(C - wrong).max() >= 1e-3 * scale
+
+
+def test_orientation_2d_non_unit_normal_known_answer():
+    """compute2DOrientationTensor divides the normal by |n|^2, not |n|
+    (BatchedComputeOrientationTensor.h:76-84), so a non-unit normal is not a
+    projection.  Two fibers, along (1,0,1)/sqrt2 and (0,1,0), worked by hand:
+    n = (0,0,2) scales to (0,0,1/2) and moves the first fiber's direction to
+    (1,0,1-1/4)/|.| = (0.8,0,0.6); n = (0,0,1/2) scales to (0,0,2) and moves
+    it to (1,0,1-4)/|.| = (1,0,-3)/sqrt10.  The second fiber is orthogonal to
+    n either way."""
+    X = np.array([[-0.25, 0.0, -0.25], [0.25, 0.0, 0.25], [0.1, -0.3, 0.0], [0.1, 0.3, 0.0]])
+    fb = np.array([[0, 1], [2, 3]], np.int32)
+    net = mf.Network(X, fb, np.array([0, 1, 2, 3], np.int32), [mf.Reaction(0, 0.01, 1e5, 1e3)])
+    o = Oracle([net])
+    e2 = np.diag([0.0, 1.0, 0.0])
+    want = {2.0: (np.array([[0.64, 0, 0.48], [0, 0, 0], [0.48, 0, 0.36]]) + e2) / 2,
+            0.5: (np.array([[0.1, 0, -0.3], [0, 0, 0], [-0.3, 0, 0.9]]) + e2) / 2,
+            1.0: (np.diag([1.0, 0.0, 0.0]) + e2) / 2}
+    for c, w in want.items():
+        om = o.orientation(normal=[0.0, 0.0, c])[0]
+        assert np.allclose(om, w, rtol=0, atol=1e-14), (c, om)
