@@ -27,18 +27,49 @@ WORKLOAD = dict(n_rve=4096, n_nodes=500, k=4, explicit_steps=5000, F_amp=0.05)
 # width of gfx942's 128 B/clk; the measured ds_read_b128 rate, ~172 B/clk per
 # CU at 2.4 GHz, is above what 128 B/clk allows) x 2.4 GHz
 LDS_SPEC = 256 * 256 * 2.4e9
-# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
-# bench (scaled by a plain copy kernel of known size), see profiles/
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+# Figures of rocprofv3 counter runs of this bench (HBM bytes per launch from
+# FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel of known size; VALU and
+# LDS instruction counts and cycles), kept beside the summaries they come
+# from.  They are replayed into the JSON line, labelled as such: counters
+# cannot be collected inside the timed run.
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+# HBM3E peak of one MI355X
+HBM_PEAK = 8.0e12
 
 
-def measured_traffic(solver):
+def replayed_counters(solver):
     try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f).get(solver)
-        return None if t is None else int(t["bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
+        with open(COUNTERS_FILE) as f:
+            return json.load(f).get(solver)
+    except (OSError, ValueError):
         return None
+
+
+def host_info():
+    """CPU model, logical CPUs and the CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def oracle_flags():
+    """Compiler and flags the CPU oracle was built with (oracle/Makefile)."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("CXXFLAGS"):
+                    return "g++ " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -62,16 +93,16 @@ def algorithmic_bytes(g, prec):
     return float((80 * Nf + 24 * Nb + 20 * M).sum())
 
 
-def time_solver(g, F_dev, sig_dev, steps, warmup):
+def time_solver(g, F_dev, sig_dev, steps, warmup, st_dev=None):
     import mumfim_amd as mf
     for _ in range(warmup):
-        g.run_device(F_dev, sig_dev)
+        g.run_device(F_dev, sig_dev, status_dev=st_dev)
         g.synchronize()
     kern_ms = []
     mf.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        g.run_device(F_dev, sig_dev)
+        g.run_device(F_dev, sig_dev, status_dev=st_dev)
         g.synchronize()
         kern_ms.append(g.last_solve_ms()[0])
     mf.device_synchronize()
@@ -83,8 +114,10 @@ def cpu_baseline(nets_fn, F, n_steps, target_s=12.0):
     """FP64 oracle on a bounded sample of the workload (OpenMP over RVEs)."""
     import mumfim_amd as mf
     from oracle.oracle import Oracle
+    # OpenMP threads: OMP_NUM_THREADS when set (the GPU host's CPU share for
+    # one GPU), else every CPU this process may use
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, 16))
+    threads = max(1, threads)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=n_steps)
     # calibrate with one RVE on one thread-slot
@@ -395,6 +428,80 @@ def bench_surrogate(args):
     print(json.dumps(res), flush=True)
 
 
+def mixed16k_batch(mf, prec):
+    """The mixed-size workload: 16384 RVEs, nodes log-uniform in [200, 4000],
+    half the fibers on the toe law, F = I + sym(U), U ~ U(-0.1, 0.1), 10000
+    ramp steps."""
+    R, S = 16384, 10000
+    u = mf.uniform(3, 11, R)
+    sizes = np.rint(np.exp(np.log(200) + u * (np.log(4000) - np.log(200)))).astype(np.int32)
+    F = strain_F(3, R, 0.1)
+    gen = mf.default_gen_params(seed=100001, nonlinear_fraction=0.5)
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, precision=prec,
+                          reaction_policy=mf.MF_REACT_PER_FIBER)
+    g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R), n_nodes=sizes)
+    return g, F, sizes, S
+
+
+def extra_mixed16k():
+    """Mixed-size batch: FP32c (1 warm-up + 2 timed batches, heuristic plan)
+    and FP64 (1 timed batch) on the whole batch; error of every RVE."""
+    import mumfim_amd as mf
+    t = time.perf_counter()
+    g, F, sizes, S = mixed16k_batch(mf, mf.MF_PREC_FP32C)
+    R = len(sizes)
+    build_s = time.perf_counter() - t
+    F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
+    sig = mf.DeviceArray((R, 6))
+    st = mf.DeviceArray((R,), np.int32)
+    wall, kms = time_solver(g, F_dev, sig, 2, 1, st)
+    s32, ok32 = sig.numpy(), bool((st.numpy() == 0).all())
+    b32 = algorithmic_bytes(g, "fp32c") * S
+    groups = g.plan()[0]
+    g.set_params(precision=mf.MF_PREC_FP64)
+    w64, k64 = time_solver(g, F_dev, sig, 1, 0, st)
+    s64, ok64 = sig.numpy(), bool((st.numpy() == 0).all())
+    g.close()
+    e = rel_err(s32, s64)
+    k32 = float(np.mean(kms))
+    return {"workload": f"{R} RVEs, nodes log-uniform [200,4000] ({int(sizes.sum())} nodes), "
+                        f"50% toe-law fibers, F=I+sym(U) U~U(-0.1,0.1), {S} ramp steps",
+            "fp32c_solves_per_s": round(2 * R / wall, 1), "fp32c_ms_per_batch": round(500 * wall, 1),
+            "fp32c_kernel_ms": round(k32, 1), "launch_groups": groups,
+            "fp64_solves_per_s": round(R / w64, 1), "fp64_ms_per_batch": round(1e3 * w64, 1),
+            "speedup_vs_fp64": round(w64 / (wall / 2), 3),
+            "rel_stress_err_vs_fp64": {"n": R, "max": float(e.max()),
+                                       "median": float(np.median(e))},
+            "status_all_ok": ok32 and ok64,
+            "lds_frac_of_spec": round(b32 / (k32 * 1e-3) / LDS_SPEC, 4),
+            "build_s": round(build_s, 1)}
+
+
+def extra_tangent1k2k():
+    """Stress + tangent batch: 4096 RVEs of 1000-2000 nodes, half the fibers
+    on the toe law, base + 12 central perturbations (h = 1e-6) per RVE in one
+    FP64 launch, FP64 differencing; one timed batch from the reference
+    state."""
+    import mumfim_amd as mf
+    R, S = 4096, 5000
+    u = mf.uniform(4, 11, R)
+    sizes = (1000 + np.floor(u * 1001)).astype(np.int32)
+    F = strain_F(4, R, 0.05)
+    gen = mf.default_gen_params(seed=200001, nonlinear_fraction=0.5)
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER,
+                          tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6)
+    g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R), n_nodes=sizes)
+    t = time.perf_counter()
+    _, Cm, st = g.run_with_tangent(F, return_status=True)
+    wall = time.perf_counter() - t
+    g.close()
+    return {"workload": f"{R} RVEs, nodes U[1000,2000], 50% toe-law fibers, base + 12 central "
+                        f"perturbations (h=1e-6) = {13 * R} solves of {S} ramp steps",
+            "fp64_tangents_per_s": round(R / wall, 1),
+            "fp64_solves_per_s": round(13 * R / wall, 1), "ms_per_batch": round(1e3 * wall, 1),
+            "status_all_ok": bool((st == 0).all()), "tangent_finite": bool(np.isfinite(Cm).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +512,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="timed batches of the FP64 solver (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-size and tangent workloads of the default run")
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
     ap.add_argument("--config", choices=["batch500", "mixed16k", "tangent1k2k", "surrogate"], default="batch500",
@@ -438,7 +547,6 @@ def main():
     # inputs resident in HBM, allocated by the library's own HIP runtime
     F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
     out = {}
-    solvers = {}
     for name, prec in (("fp32c", mf.MF_PREC_FP32C), ("fp64", mf.MF_PREC_FP64)):
         if args.only and args.only != name:
             continue
@@ -447,13 +555,14 @@ def main():
         g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, R))
         log(f"[bench] {name}: batch of {R} RVEs built in {time.perf_counter() - t:.1f}s")
         sig = mf.DeviceArray((R, 6), np.float64)
+        st = mf.DeviceArray((R,), np.int32)
         steps = args.steps if name == "fp32c" else (args.fp64_steps or args.steps)
-        wall, kms = time_solver(g, F_dev, sig, steps, args.warmup)
+        wall, kms = time_solver(g, F_dev, sig, steps, args.warmup, st)
         log(f"[bench] {name}: {steps} batches in {wall:.3f}s, kernel ms {kms}")
-        solvers[name] = g
-        out[name] = dict(wall=wall, steps=steps, kernel_ms=kms,
-                         sigma=sig.numpy(),
+        out[name] = dict(wall=wall, steps=steps, kernel_ms=kms, sigma=sig.numpy(),
+                         status_ok=bool((st.numpy() == mf.MF_STATUS_OK).all()),
                          bytes=algorithmic_bytes(g, name))
+        g.close()
     head = "fp32c" if "fp32c" in out else "fp64"
     h = out[head]
     value = R * h["steps"] / h["wall"]
@@ -462,6 +571,35 @@ def main():
     lds_meas = mf.measure_lds_bandwidth()
     hbm_meas = mf.measure_hbm_bandwidth(1 << 30)
     achieved = h["bytes"] * S / (kern_ms * 1e-3)
+    # LDS ceiling: 256 B/clk/CU x 256 CUs x 2.4 GHz, unless the probe (which
+    # spends ~2 VALU instructions per 1 KiB wave read, so issue does not bind
+    # it) measures more
+    peak = max(LDS_SPEC, lds_meas)
+    ctr = replayed_counters(head) or {}
+    traffic = ctr.get("bytes_per_launch")
+    roof = {
+        "bound": "lds",
+        "achieved": round(achieved / 1e9, 1),
+        "peak": round(peak / 1e9, 1),
+        "unit": "GB/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": None if traffic is None else int(traffic),
+        "traffic_source": None if traffic is None else
+        f"replayed from {os.path.relpath(COUNTERS_FILE, ROOT)} ({ctr.get('source', '')})",
+        "kernel_ms": round(kern_ms, 3),
+        "algorithmic_bytes_per_launch": h["bytes"] * S,
+        "peak_source": "LDS spec 256 B/clk/CU x 256 CU x 2.4 GHz" if peak == LDS_SPEC
+        else "LDS probe (ds_read_b128, all CUs)",
+        "lds_probe_GBps": round(lds_meas / 1e9, 1),
+        "lds_probe_B_per_clk_per_CU": round(lds_meas / (256 * 2.4e9), 1),
+        "hbm_copy_measured_GBps": round(hbm_meas / 1e9, 1),
+    }
+    if traffic is not None:
+        roof["hbm_frac_of_8TBps"] = round(traffic / (kern_ms * 1e-3) / HBM_PEAK, 5)
+    if "valu_issue" in ctr:
+        roof["valu_issue"] = ctr["valu_issue"]
+        roof["valu_issue_note"] = ("(SQ_INSTS_VALU + SQ_INSTS_LDS) / (256 CU x kernel cycles), "
+                                   "replayed; 1.0 is one wave-instruction per CU per clock")
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -483,19 +621,8 @@ def main():
                       if head == "fp32c" else "fp64 reference arithmetic",
             "parallelism": "1 GPU, one workgroup per RVE solve",
         },
-        "roofline": {
-            "bound": "lds",
-            "achieved": round(achieved / 1e9, 1),
-            "peak": round(lds_meas / 1e9, 1),
-            "unit": "GB/s",
-            "frac": round(achieved / lds_meas, 4),
-            "traffic": measured_traffic(head),
-            "peak_spec": round(LDS_SPEC / 1e9, 1),
-            "frac_of_spec": round(achieved / LDS_SPEC, 4),
-            "kernel_ms": round(kern_ms, 3),
-            "algorithmic_bytes_per_launch": h["bytes"] * S,
-            "hbm_copy_measured_GBps": round(hbm_meas / 1e9, 1),
-        },
+        "roofline": roof,
+        "status_all_ok": h["status_ok"],
     }
     if "fp64" in out and "fp32c" in out:
         s64, s32 = out["fp64"]["sigma"], out["fp32c"]["sigma"]
@@ -503,9 +630,18 @@ def main():
         v64 = R * out["fp64"]["steps"] / out["fp64"]["wall"]
         res["fp64"] = {"value": round(v64, 3), "ms_per_step":
                        round(1e3 * out["fp64"]["wall"] / out["fp64"]["steps"], 3),
-                       "kernel_ms": round(float(np.mean(out["fp64"]["kernel_ms"])), 3)}
+                       "kernel_ms": round(float(np.mean(out["fp64"]["kernel_ms"])), 3),
+                       "status_all_ok": out["fp64"]["status_ok"]}
         res["speedup_vs_fp64"] = round(value / v64, 3)
         res["rel_stress_err_vs_fp64"] = {"max": float(err.max()), "median": float(np.median(err))}
+        # the metric is quoted at <= 1e-4 relative stress error: a run that
+        # misses it (or has a failed solve) reports no value
+        if not (err.max() <= 1e-4) or not h["status_ok"] or not out["fp64"]["status_ok"]:
+            res["value"] = None
+            res["invalid_reason"] = "stress error above 1e-4 or a solve status not OK"
+    elif not h["status_ok"]:
+        res["value"] = None
+        res["invalid_reason"] = "a solve status not OK"
     if not args.no_cpu:
         t = time.perf_counter()
         nets_fn = lambda n: [mf.generate_network(seed=1 + r, n_nodes=WORKLOAD["n_nodes"])
@@ -516,11 +652,23 @@ def main():
         res["cpu_baseline"] = {
             "value": round(cv, 3), "unit": "RVE solves/s", "cores": threads, "kind": "port",
             "sample": f"first {n} RVEs of the workload ({S} steps each), FP64 oracle, "
-                      f"OpenMP {threads} threads, {dt:.1f}s",
+                      f"OpenMP {threads} threads (one RVE per thread, dynamic schedule), "
+                      f"{dt:.1f}s",
+            "omp_threads": threads, "compiler": oracle_flags(), **host_info(),
             "fp64_gpu_vs_cpu_max_rel_err": perr,
         }
         log(f"[bench] cpu baseline {cv:.2f} solves/s on {threads} threads "
             f"({time.perf_counter() - t:.1f}s)")
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("mixed16k", extra_mixed16k), ("tangent1k2k", extra_tangent1k2k)):
+            t = time.perf_counter()
+            try:
+                extras[name] = fn()
+            except Exception as e:  # an extra never takes the headline down
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] extra {name} in {time.perf_counter() - t:.1f}s: {extras[name]}")
+        res["extra"] = extras
     print(json.dumps(res), flush=True)
 
 

@@ -372,7 +372,9 @@ class BatchedFiberRVEAnalysisExplicit:
     """Host mirror of the reference batched analysis over the C ABI."""
 
     def __init__(self, networks=None, params=None, *, generated=None, n_nodes=None):
-        self.params = params if params is not None else default_params()
+        # own copy: set_params() must not change the caller's struct
+        self.params = (mf_solver_params.from_buffer_copy(params) if params is not None
+                       else default_params())
         h = C.c_void_p()
         if generated is not None:
             g, n_rve = generated

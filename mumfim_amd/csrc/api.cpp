@@ -70,7 +70,10 @@ void make_react_dev(const Reaction &q, ReactDev &d) {
 // Almansi-strain perturbation F_i = sqrt((I - 2e_i)^-1) with e_i the unit
 // strain of Voigt component i scaled by h (engineering shear), as in
 // BatchedFiberRVEAnalysisExplicit.h:340-389 and RVEAnalysis.cc:15-64.
+// Rounded without contraction, operation for operation as the oracle does,
+// so the FP64 tangent's perturbed solves see the same F bits.
 void perturbation(int i, double h, double Fp[9]) {
+#pragma clang fp contract(off)
   double D1 = std::sqrt(1.0 / (1 - 2 * h));
   const double l1 = 1;
   const double l2 = (1 + h) / (1 - h * h);
@@ -89,6 +92,7 @@ void perturbation(int i, double h, double Fp[9]) {
 }
 
 void matmul3(const double A[9], const double B[9], double C[9]) {
+#pragma clang fp contract(off)
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       double s = 0;

@@ -9,25 +9,29 @@ namespace mf {
 namespace kern {
 
 // ---- small kernels -------------------------------------------------------------
-// LDS read-bandwidth probe: each thread streams float4 reads over a 64 KiB
-// tile in lane-consecutive (conflict-free) order.
-__global__ __launch_bounds__(256) void lds_probe_kernel(float *out, int iters) {
+// LDS read-bandwidth probe.  Per iteration each thread reads 16 float4 at
+// one lane-consecutive (conflict-free) base with immediate offsets 4 KiB
+// apart over a 64 KiB tile; the base rotates by a wave width every iteration
+// so no load is loop-invariant.  Every dword is folded with 3-input integer
+// adds (v_add3_u32), two VALU instructions per 1 KiB wave read, half the
+// issue slots a 256 B/clk LDS leaves per read, so the probe is bound by the
+// LDS, not by issue.
+__global__ __launch_bounds__(256) void lds_probe_kernel(uint32_t *out, int iters) {
   __shared__ float4 tile[4096];
   const int t = threadIdx.x;
   for (int i = t; i < 4096; i += 256) tile[i] = make_float4(i, 1.0f, 2.0f, 3.0f);
   __syncthreads();
-  float4 acc = make_float4(0, 0, 0, 0);
+  uint32_t a0 = 0, a1 = 0;
   for (int it = 0; it < iters; ++it) {
+    const float4 *p = tile + ((it * 64 + t) & 255);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const float4 v = tile[(k * 256 + t + it) & 4095];
-      acc.x += v.x;
-      acc.y += v.y;
-      acc.z += v.z;
-      acc.w += v.w;
+      const float4 v = p[k * 256];
+      a0 = a0 + __float_as_uint(v.x) + __float_as_uint(v.y);
+      a1 = a1 + __float_as_uint(v.z) + __float_as_uint(v.w);
     }
   }
-  out[blockIdx.x * 256 + t] = acc.x + acc.y + acc.z + acc.w;
+  out[blockIdx.x * 256 + t] = a0 + a1;
 }
 
 __global__ void copy_probe_kernel(const float4 *in, float4 *out, int64_t n) {
@@ -258,8 +262,8 @@ hipError_t lds_probe(double *bytes_per_s) {
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   const int blocks = cus * 8, iters = 4096;
-  float *out = nullptr;
-  e = hipMalloc(&out, sizeof(float) * blocks * 256);
+  uint32_t *out = nullptr;
+  e = hipMalloc(&out, sizeof(uint32_t) * blocks * 256);
   if (e != hipSuccess) return e;
   hipEvent_t a, b;
   (void)hipEventCreate(&a);

@@ -52,7 +52,8 @@ class Oracle:
     """Mirror of BatchedFiberRVEAnalysisExplicit on the CPU oracle."""
 
     def __init__(self, networks, params=None):
-        self.params = params if params is not None else default_params()
+        self.params = (mf_solver_params.from_buffer_copy(params) if params is not None
+                       else default_params())
         self.nets = networks
         arr = (mf_network * len(networks))(*[n.to_c() for n in networks])
         h = C.c_void_p()

@@ -168,3 +168,15 @@ def test_header_is_plain_c_and_links(tmp_path):
                         str(src), "-o", str(tmp_path / "use"), "-L", lib, "-lmumfim_amd",
                         f"-Wl,-rpath,{lib}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_wrappers_copy_the_params_struct():
+    """set_params on one batch never changes the struct the caller passed
+    (and may pass to another batch or to the oracle)."""
+    import mumfim_amd as mf
+    from oracle.oracle import Oracle
+    p = mf.default_params(fd_step=1e-6)
+    net = mf.generate_network(seed=3, n_nodes=60)
+    o = Oracle([net], p)
+    o.set_params(fd_step=1e-5)
+    assert p.fd_step == 1e-6 and o.params.fd_step == 1e-5
