@@ -186,6 +186,25 @@ int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
 int mf_batch_run_device_async(mf_batch *b, const double *F_dev, int32_t nF,
                               double *sigma_dev, int32_t update_coords,
                               int32_t *status_dev);
+/* Allocation-free run.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It is
+ * mf_batch_run_device_async with the solves' scratch (F staging for a
+ * broadcast F, per-solve step counters, statuses when status_dev is NULL,
+ * and the global state of RVEs beyond the LDS kernels) taken from a
+ * library-wide device array of MF_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; it calls no allocating routine.  Scratch needed:
+ * align256(72 n) + 2 align256(4 n) + 8 * (global-state doubles of the batch's
+ * plan, 96 bytes per node of each RVE that runs on the global-state kernel).
+ * A batch needing more returns MF_ERR_INVALID.  Energy diagnostics are
+ * rejected (MF_ERR_UNSUPPORTED).  Nothing in the scratch needs clearing: the
+ * kernels write every word they read.  All calls on one GPU share its array,
+ * so calls that overlap on one GPU (two batches on different streams, or
+ * threads) race: serialise them. */
+#define MF_NOALLOC_SCRATCH_BYTES (64 << 20)
+int mf_batch_run_device_noalloc(mf_batch *b, const double *F_dev, int32_t nF,
+                                double *sigma_dev, int32_t update_coords,
+                                int32_t *status_dev);
 int mf_batch_synchronize(mf_batch *b);
 /* HIP stream (hipStream_t) the batch queues its work on. */
 void *mf_batch_stream(mf_batch *b);

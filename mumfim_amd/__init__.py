@@ -104,6 +104,7 @@ SIGNATURES = {
     "mf_batch_tangent": (C.c_int, [_P, _D, _IP]),
     "mf_batch_run_with_tangent": (C.c_int, [_P, _D, _I, _D, _D, _IP]),
     "mf_batch_run_device_async": (C.c_int, [_P, _P, _I, _P, _I, _P]),
+    "mf_batch_run_device_noalloc": (C.c_int, [_P, _P, _I, _P, _I, _P]),
     "mf_batch_synchronize": (C.c_int, [_P]),
     "mf_batch_stream": (C.c_void_p, [_P]),
     "mf_batch_last_solve_ms": (C.c_int, [_P, _D, _IP]),

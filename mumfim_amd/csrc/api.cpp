@@ -8,6 +8,7 @@
 #include <atomic>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <tuple>
 #include <chrono>
@@ -220,6 +221,8 @@ struct mf_batch {
   std::vector<size_t> id_group_lds[2];              // LDS bytes per group
   std::vector<int64_t> id_group_gs[2];              // global state per solve
   DevBuf<double> gstate;  // global-state variants' per-solve scratch
+  size_t id_gtotal[2] = {0, 0};  // doubles of global state of a one-solve-per-RVE run
+  int device = 0;                // HIP device the batch lives on
   DevBuf<double> lastF_dev;
   bool lastF_on_device = false;
   // energies of the last run() (FP64 energy diagnostics), 4 per RVE
@@ -475,6 +478,7 @@ int upload_batch(mf_batch *b) {
   std::vector<double> ones(R, 1.0);
   MF_HIP(b->V.upload(ones, s));
   MF_HIP(b->cur_stress.alloc(6 * (size_t)R));
+  MF_HIP(b->lastF_dev.alloc(9 * (size_t)R));
   MF_HIP(hipStreamSynchronize(s));
   return MF_OK;
 }
@@ -483,7 +487,8 @@ int build_identity_tables(mf_batch *b);
 void fill_args(mf_batch *b, SolveArgs &a);
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
-                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride);
+                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride,
+                  double *gstate_base = nullptr);
 // doubles of global state per solve of a group of variant v whose largest
 // RVE has N nodes (global-state variants; 0 otherwise)
 int64_t group_gstride(const Variant &v, int N) {
@@ -530,6 +535,7 @@ int create_from_networks(std::vector<Network> &nets, const mf_solver_params *par
     set_error("no HIP device available");
     return MF_ERR_HIP;
   }
+  MF_HIP(hipGetDevice(&b->device));
   MF_HIP(hipStreamCreate(&b->stream));
   MF_HIP(hipEventCreate(&b->ev0));
   MF_HIP(hipEventCreate(&b->ev1));
@@ -728,6 +734,7 @@ int build_identity_tables(mf_batch *b) {
       i = j;
     }
     // allocated here, so a captured run launch never allocates
+    b->id_gtotal[prec] = gtotal;
     MF_HIP(b->gstate.ensure(gtotal));
     MF_HIP(b->id_order[prec].upload(order, s));
     // flags: [0..R) update=false, [R..2R) update=true
@@ -745,17 +752,20 @@ int build_identity_tables(mf_batch *b) {
 
 // Launches groups (variant, solve count, LDS bytes) of consecutive entries
 // of `order`; group 0 runs on the batch stream, the others on the auxiliary
-// streams, all after ev0 and joined back before ev1.
+// streams, all after ev0 and joined back before ev1.  gstate_base, when
+// given, holds the groups' global state (the caller sized it); otherwise the
+// batch's own buffer is used and grown as needed.
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
-                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride) {
+                  const std::vector<size_t> &lds, const std::vector<int64_t> &gstride,
+                  double *gstate_base) {
   int nv = 0;
   const Variant *vs = variants(&nv);
   hipStream_t s = b->stream;
   size_t gtotal = 0;
   for (size_t gi = 0; gi < groups.size(); ++gi)
     gtotal += (size_t)groups[gi].second * (size_t)gstride[gi];
-  if (gtotal > b->gstate.n) {
+  if (!gstate_base && gtotal > b->gstate.n) {
     if (b->capturing) {
       set_error("global-state scratch would grow inside a captured launch");
       return MF_ERR_STATE;
@@ -778,7 +788,7 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
       used[k] = true;
     }
     a.order = order + off;
-    a.gstate = gstride[gi] ? b->gstate.p + goff : nullptr;
+    a.gstate = gstride[gi] ? (gstate_base ? gstate_base : b->gstate.p) + goff : nullptr;
     a.gstride = gstride[gi];
     goff += (size_t)groups[gi].second * (size_t)gstride[gi];
     hipError_t e = launch_solve(vs[groups[gi].first], a, groups[gi].second, lds[gi], st);
@@ -820,9 +830,21 @@ int check_F(mf_batch *b, int nF, int update) {
   return MF_OK;
 }
 
+// Device scratch of the allocation-free run: F staging for a broadcast F,
+// per-solve step counters and statuses, global state.
+struct RunScratch {
+  double *F_stage;
+  int32_t *steps;
+  int32_t *status;
+  double *gstate;
+};
+
 // run() with F on the device and one solve per RVE: no host round trip.
+// With `na` every buffer the launches write besides the caller's comes from
+// it or was allocated with the batch, and nothing is allocated.
 int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
-                    double *sigma_dev, int32_t *status_dev) {
+                    double *sigma_dev, int32_t *status_dev,
+                    const RunScratch *na = nullptr) {
   const int R = b->R;
   hipStream_t s = b->stream;
   const bool ramp = b->p.mode == MF_MODE_RAMP;
@@ -836,17 +858,21 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
   }
   const double *Fs = F_dev;
   if (nF == 1) {
-    hipError_t e = launch_broadcast_F(F_dev, b->id_F.p, R, s);
+    double *Fb = na ? na->F_stage : b->id_F.p;
+    hipError_t e = launch_broadcast_F(F_dev, Fb, R, s);
     if (e != hipSuccess) {
       set_error(hipGetErrorString(e));
       return MF_ERR_HIP;
     }
-    Fs = b->id_F.p;
+    Fs = Fb;
   }
-  MF_HIP(b->sigma.ensure(6 * (size_t)R));
-  MF_HIP(b->status.ensure(R));
+  if (!na) {
+    MF_HIP(b->sigma.ensure(6 * (size_t)R));
+    MF_HIP(b->status.ensure(R));
+  }
   SolveArgs a;
   fill_args(b, a);
+  if (na) a.steps = na->steps;
   const int c = b->cur;
   a.x_in = b->xs[c].p;
   a.u_in = b->us[c].p;
@@ -856,15 +882,21 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
   a.solve_F = Fs;
   a.solve_flags = b->id_flags[prec].p + (update ? R : 0);
   a.sigma = sigma_dev ? sigma_dev : b->sigma.p;
-  a.status = status_dev ? status_dev : b->status.p;
+  a.status = status_dev ? status_dev : na ? na->status : b->status.p;
   b->have_energy = prec == MF_PREC_FP64 && b->p.energy_diagnostics;
   if (b->have_energy) {
+    if (na) {
+      set_error("energy diagnostics need a library buffer; the allocation-free run "
+                "does not track them");
+      return MF_ERR_UNSUPPORTED;
+    }
     MF_HIP(b->energy.ensure(4 * (size_t)R));
     a.energy = b->energy.p;
   }
   {
     int e = launch_groups(b, a, b->id_order[prec].p, b->id_groups[prec],
-                          b->id_group_lds[prec], b->id_group_gs[prec]);
+                          b->id_group_lds[prec], b->id_group_gs[prec],
+                          na ? na->gstate : nullptr);
     if (e != MF_OK) return e;
   }
   if (!ramp) b->at_reference = false;  // relax runs keep the trial x (Q5)
@@ -876,7 +908,7 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
     if (ramp) {
       b->cur ^= 1;
       b->have_prev = true;
-      MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));
+      if (!na) MF_HIP(b->lastF_dev.ensure(9 * (size_t)R));  // allocated with the batch
       MF_HIP(hipMemcpyAsync(b->lastF_dev.p, Fs, sizeof(double) * 9 * R,
                             hipMemcpyDeviceToDevice, s));
       b->lastF_on_device = true;
@@ -1119,6 +1151,39 @@ int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
   MF_HIP(hipStreamSynchronize(s));
   return MF_OK;
 }
+
+// ---- scratch of the allocation-free run ------------------------------------
+// Device memory of mf_batch_run_device_noalloc, held by every GPU the library
+// runs on until the process ends.  Host code never uses the array's name as
+// an address: each device's copy is located once with hipGetSymbolAddress and
+// that pointer is what kernels and copies receive.
+static __device__ __align__(256) unsigned char scratch[MF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on the current device (looked up on first use).
+int scratch_address(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index out of range");
+    return MF_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return MF_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+  return MF_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -1480,6 +1545,49 @@ int mf_batch_run_device_async(mf_batch *b, const double *F_dev, int32_t nF,
   if (e != MF_OK) return e;
   if (!b->use_graph) return run_device_fast(b, F_dev, nF, update_coords, sigma_dev, status_dev);
   return run_device_graph(b, F_dev, nF, update_coords, sigma_dev, status_dev);
+}
+
+int mf_batch_run_device_noalloc(mf_batch *b, const double *F_dev, int32_t nF,
+                                double *sigma_dev, int32_t update_coords,
+                                int32_t *status_dev) {
+  if (!b || !F_dev || !sigma_dev) {
+    set_error("mf_batch_run_device_noalloc: NULL argument");
+    return MF_ERR_INVALID;
+  }
+  int e = check_F(b, nF, update_coords);
+  if (e != MF_OK) return e;
+  const int R = b->R, prec = b->p.precision;
+  if (prec == MF_PREC_FP64 && b->p.energy_diagnostics) {
+    set_error("mf_batch_run_device_noalloc: energy diagnostics are not supported "
+              "(they need a library buffer)");
+    return MF_ERR_UNSUPPORTED;
+  }
+  // layout: F staging (9R doubles), step counters (R), statuses (R), global state
+  const size_t off_steps = align256(sizeof(double) * 9 * (size_t)R);
+  const size_t off_status = off_steps + align256(sizeof(int32_t) * (size_t)R);
+  const size_t off_gstate = off_status + align256(sizeof(int32_t) * (size_t)R);
+  const size_t need = off_gstate + sizeof(double) * b->id_gtotal[prec];
+  if (need > (size_t)MF_NOALLOC_SCRATCH_BYTES) {
+    set_error("mf_batch_run_device_noalloc: the batch needs " + std::to_string(need) +
+              " bytes of scratch, above the maximum of " +
+              std::to_string((size_t)MF_NOALLOC_SCRATCH_BYTES) +
+              " (use mf_batch_run_device_async)");
+    return MF_ERR_INVALID;
+  }
+  int prev = -1;
+  MF_HIP(hipGetDevice(&prev));
+  if (prev != b->device) MF_HIP(hipSetDevice(b->device));
+  unsigned char *base = nullptr;
+  e = scratch_address(b->device, &base);
+  if (e == MF_OK) {
+    RunScratch na{reinterpret_cast<double *>(base),
+                  reinterpret_cast<int32_t *>(base + off_steps),
+                  reinterpret_cast<int32_t *>(base + off_status),
+                  reinterpret_cast<double *>(base + off_gstate)};
+    e = run_device_fast(b, F_dev, nF, update_coords, sigma_dev, status_dev, &na);
+  }
+  if (prev != b->device) (void)hipSetDevice(prev);
+  return e;
 }
 
 int mf_batch_synchronize(mf_batch *b) {
