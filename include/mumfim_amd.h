@@ -131,6 +131,13 @@ typedef struct mf_solver_params {
   int32_t energy_diagnostics;   /* 1: track energies                           */
   int32_t fp32_kernel;          /* MF_FP32_KERNEL_*: FP32c kernel family      */
   double energy_check_eps;      /* 0: no check                                 */
+  /* Ramp mode, FP32c register kernels: each solve of run() is split into
+   * this many step chunks, queued all first chunks before any second one,
+   * so the end of a launch drains in short pieces (results are bitwise the
+   * same).  0 = planner's choice from the batch size and residency, 1 = no
+   * chunks, up to 16. */
+  int32_t chunks;
+  int32_t pad_;
 } mf_solver_params;
 
 /* FP32c kernel families.  Node-centric kernels gather each free node's
@@ -256,7 +263,9 @@ int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
  * 32 one linear and one nonlinear reaction, 64 fiber-parallel, 128
  * fiber-streamed (bits 8-11: fibers per thread), 8192 FP64 with the state in
  * global memory (RVEs no other variant holds, in either precision), 16384
- * FP64 evaluating each fiber once (bitwise the node-centric FP64 result).
+ * FP64 evaluating each fiber once (bitwise the node-centric FP64 result);
+ * bits 24-28: step chunks per solve of run() when more than one
+ * (params.chunks).
  * Returns the number of distinct launch groups. */
 int mf_batch_plan(mf_batch *b, int32_t *plan);
 

@@ -54,7 +54,7 @@ class mf_solver_params(C.Structure):
                 ("crit_time_scale_factor", C.c_double),
                 ("tolerance", C.c_double), ("fd_step", C.c_double),
                 ("energy_diagnostics", C.c_int32), ("fp32_kernel", C.c_int32),
-                ("energy_check_eps", C.c_double)]
+                ("energy_check_eps", C.c_double), ("chunks", C.c_int32), ("pad_", C.c_int32)]
 
 
 class mf_gen_params(C.Structure):

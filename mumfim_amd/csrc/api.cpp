@@ -122,7 +122,8 @@ bool params_valid(const mf_solver_params &p) {
          (p.fp32_kernel == MF_FP32_KERNEL_AUTO || p.fp32_kernel == MF_FP32_KERNEL_NODE ||
           p.fp32_kernel == MF_FP32_KERNEL_FIBER) &&
          (p.block_size == 0 || p.block_size == 256 || p.block_size == 512 ||
-          p.block_size == 640 || p.block_size == 768 || p.block_size == 1024);
+          p.block_size == 640 || p.block_size == 768 || p.block_size == 1024) &&
+         p.chunks >= 0 && p.chunks <= 16;
 }
 
 template <class F>
@@ -199,7 +200,7 @@ struct mf_batch {
     double *sigma;
     int32_t *status;
     int cur, prec, version;
-    static constexpr int kScratch = 8;
+    static constexpr int kScratch = 10;
     const void *scratch[kScratch];  // library buffers the captured launches use
     hipGraphExec_t exec;
   };
@@ -220,6 +221,11 @@ struct mf_batch {
   std::vector<std::pair<int, int>> id_groups[2];  // (variant, count) in order
   std::vector<size_t> id_group_lds[2];              // LDS bytes per group
   std::vector<int64_t> id_group_gs[2];              // global state per solve
+  std::vector<int> id_group_chunks[2];              // step chunks per solve
+  // chunk state of chunked groups (one-solve-per-RVE runs) and the per-RVE
+  // chunk counters (zero between launches)
+  DevBuf<float> cstate;
+  DevBuf<int32_t> cflag;
   DevBuf<double> gstate;  // global-state variants' per-solve scratch
   size_t id_gtotal[2] = {0, 0};  // doubles of global state of a one-solve-per-RVE run
   int device = 0;                // HIP device the batch lives on
@@ -488,11 +494,42 @@ void fill_args(mf_batch *b, SolveArgs &a);
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
                   const std::vector<size_t> &lds, const std::vector<int64_t> &gstride,
-                  double *gstate_base = nullptr);
+                  double *gstate_base = nullptr, const std::vector<int> *chunks = nullptr);
 // doubles of global state per solve of a group of variant v whose largest
 // RVE has N nodes (global-state variants; 0 otherwise)
 int64_t group_gstride(const Variant &v, int N) {
   return (v.flags & kGlobalState) ? 12 * (int64_t)((N + 1) & ~1) : 0;
+}
+
+// Step chunks per solve for a launch group of n solves of variant v (ramp
+// mode, FP32c register kernels): with S workgroups resident on the device,
+// n solves of equal length take ceil(n / S) rounds, the last partly empty;
+// c chunks per solve take ceil(n c / S) rounds of 1/c the length.  The
+// count minimising that (plus 1% per extra chunk for the state round trip)
+// is chosen, and only when n >= 2 S, so a chunk's predecessor has finished
+// long before it starts.  params.chunks > 0 forces the count.
+int plan_chunks(const mf_batch *b, const Variant &v, size_t lds, int n) {
+  if (b->p.mode != MF_MODE_RAMP || v.prec != MF_PREC_FP32C || !(v.flags & kEllInRegs))
+    return 1;
+  const int force = b->p.chunks;
+  if (force > 0) return std::max(1, std::min(force, b->p.n_steps));
+  const int occ = chunk_occupancy(v, lds);
+  int cus = 0;
+  if (occ <= 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) !=
+                      hipSuccess || cus <= 0)
+    return 1;
+  const long S = (long)occ * cus;
+  if (n < 2 * S) return 1;
+  int best = 1;
+  double best_t = 1e300;
+  for (int c = 1; c <= 4 && c <= b->p.n_steps; ++c) {
+    const double t = (double)((n * (long)c + S - 1) / S) / c * (1.0 + 0.01 * (c - 1));
+    if (t < best_t - 1e-12) {
+      best_t = t;
+      best = c;
+    }
+  }
+  return best;
 }
 
 int create_from_networks(std::vector<Network> &nets, const mf_solver_params *params,
@@ -710,13 +747,17 @@ int build_identity_tables(mf_batch *b) {
   MF_HIP(b->id_rve.upload(ids, s));
   MF_HIP(b->id_F.alloc(9 * (size_t)R));
   MF_HIP(b->steps.ensure(R));
+  size_t ctotal = 0;  // chunk state of both precisions' plans (the larger is kept)
+  size_t cneed = 0;
   for (int prec = 0; prec < 2; ++prec) {
+    ctotal = 0;
     const std::vector<int32_t> order = launch_order(b, prec, R, [](int i) { return i; });
     b->id_groups[prec].clear();
     b->id_group_lds[prec].clear();
     b->id_group_gs[prec].clear();
     int nv = 0;
     const Variant *vs = variants(&nv);
+    b->id_group_chunks[prec].clear();
     size_t gtotal = 0;
     for (int i = 0; i < R;) {
       int v = b->plan[prec][order[i]], j = i;
@@ -731,11 +772,15 @@ int build_identity_tables(mf_batch *b) {
       b->id_group_lds[prec].push_back(L);
       b->id_group_gs[prec].push_back(group_gstride(vs[v], maxN));
       gtotal += (size_t)(j - i) * (size_t)b->id_group_gs[prec].back();
+      const int c = plan_chunks(b, vs[v], L, j - i);
+      b->id_group_chunks[prec].push_back(c);
+      if (c > 1) ctotal += (size_t)(j - i) * (size_t)chunk_stride(vs[v].block, vs[v].npt);
       i = j;
     }
     // allocated here, so a captured run launch never allocates
     b->id_gtotal[prec] = gtotal;
     MF_HIP(b->gstate.ensure(gtotal));
+    cneed = std::max(cneed, ctotal);
     MF_HIP(b->id_order[prec].upload(order, s));
     // flags: [0..R) update=false, [R..2R) update=true
     std::vector<int32_t> fl(2 * (size_t)R);
@@ -745,6 +790,13 @@ int build_identity_tables(mf_batch *b) {
       fl[R + r] = kWriteX | kWriteU;
     }
     MF_HIP(b->id_flags[prec].upload(fl, s));
+  }
+  if (cneed > 0) {
+    MF_HIP(b->cstate.ensure(cneed));
+    if (b->cflag.n < (size_t)R) {
+      MF_HIP(b->cflag.alloc(R));
+      MF_HIP(hipMemsetAsync(b->cflag.p, 0, sizeof(int32_t) * R, s));
+    }
   }
   MF_HIP(hipStreamSynchronize(s));
   return MF_OK;
@@ -758,7 +810,7 @@ int build_identity_tables(mf_batch *b) {
 int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
                   const std::vector<std::pair<int, int>> &groups,
                   const std::vector<size_t> &lds, const std::vector<int64_t> &gstride,
-                  double *gstate_base) {
+                  double *gstate_base, const std::vector<int> *chunks) {
   int nv = 0;
   const Variant *vs = variants(&nv);
   hipStream_t s = b->stream;
@@ -772,7 +824,7 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
     }
     MF_HIP(b->gstate.ensure(gtotal));
   }
-  size_t goff = 0;
+  size_t goff = 0, coff = 0;
   // under capture the fork uses its own event and the timing events stay
   // outside the graph
   hipEvent_t fork = b->capturing ? b->ev_fork : b->ev0;
@@ -791,7 +843,22 @@ int launch_groups(mf_batch *b, SolveArgs a, const int32_t *order,
     a.gstate = gstride[gi] ? (gstate_base ? gstate_base : b->gstate.p) + goff : nullptr;
     a.gstride = gstride[gi];
     goff += (size_t)groups[gi].second * (size_t)gstride[gi];
-    hipError_t e = launch_solve(vs[groups[gi].first], a, groups[gi].second, lds[gi], st);
+    const Variant &vg = vs[groups[gi].first];
+    const int nc = chunks ? (*chunks)[gi] : 1;
+    a.n_chunks = nc;
+    a.group_n = groups[gi].second;
+    a.chunk_steps = nc > 1 ? (b->p.n_steps + nc - 1) / nc : 0;
+    a.cstride = nc > 1 ? chunk_stride(vg.block, vg.npt) : 0;
+    a.cstate = nc > 1 ? b->cstate.p + coff : nullptr;
+    a.cflag = nc > 1 ? b->cflag.p : nullptr;
+    if (nc > 1) {
+      coff += (size_t)groups[gi].second * (size_t)a.cstride;
+      if (coff > b->cstate.n || !b->cflag.p) {
+        set_error("step-chunk state not allocated for this launch");
+        return MF_ERR_STATE;
+      }
+    }
+    hipError_t e = launch_solve(vg, a, groups[gi].second * nc, lds[gi], st);
     if (e != hipSuccess) {
       set_error(std::string("solve launch: ") + hipGetErrorString(e));
       return MF_ERR_HIP;
@@ -894,9 +961,10 @@ int run_device_fast(mf_batch *b, const double *F_dev, int nF, int update,
     a.energy = b->energy.p;
   }
   {
+    // the allocation-free run has no chunk state: one piece per solve
     int e = launch_groups(b, a, b->id_order[prec].p, b->id_groups[prec],
                           b->id_group_lds[prec], b->id_group_gs[prec],
-                          na ? na->gstate : nullptr);
+                          na ? na->gstate : nullptr, na ? nullptr : &b->id_group_chunks[prec]);
     if (e != MF_OK) return e;
   }
   if (!ramp) b->at_reference = false;  // relax runs keep the trial x (Q5)
@@ -936,7 +1004,7 @@ int run_device_graph(mf_batch *b, const double *F_dev, int nF, int update, doubl
   // was reallocated since the capture makes the graph stale
   const void *scratch[mf_batch::Graph::kScratch] = {
       b->sigma.p, b->status.p, b->energy.p, b->lastF_dev.p,
-      b->eta_rve.p, b->steps.p, b->gstate.p, b->cur_stress.p};
+      b->eta_rve.p, b->steps.p, b->gstate.p, b->cur_stress.p, b->cstate.p, b->cflag.p};
   mf_batch::Graph *g = nullptr;
   for (auto &x : b->graphs)
     if (x.F == F_dev && x.nF == nF && x.update == update && x.sigma == sigma_dev &&
@@ -1825,13 +1893,27 @@ int mf_batch_plan(mf_batch *b, int32_t *plan) {
   int nv = 0;
   const Variant *vs = variants(&nv);
   const int prec = b->p.precision;
+  // step chunks of each RVE's launch group (one-solve-per-RVE runs)
+  std::vector<int> nch(b->R, 1);
+  {
+    std::vector<int32_t> order(b->R);
+    if (b->R > 0)
+      MF_HIP(hipMemcpy(order.data(), b->id_order[prec].p, sizeof(int32_t) * b->R,
+                       hipMemcpyDeviceToHost));
+    int off = 0;
+    for (size_t gi = 0; gi < b->id_groups[prec].size(); ++gi) {
+      for (int k = 0; k < b->id_groups[prec][gi].second; ++k)
+        nch[order[off + k]] = b->id_group_chunks[prec][gi];
+      off += b->id_groups[prec][gi].second;
+    }
+  }
   std::vector<int> seen;
   for (int r = 0; r < b->R; ++r) {
     const Variant &v = vs[b->plan[prec][r]];
     if (plan) {
       plan[4 * r + 0] = v.block;
       plan[4 * r + 1] = v.npt;
-      plan[4 * r + 2] = v.flags;
+      plan[4 * r + 2] = v.flags | (nch[r] > 1 ? nch[r] << 24 : 0);
       plan[4 * r + 3] = (int32_t)b->lds[prec][r];
     }
     if (std::find(seen.begin(), seen.end(), b->plan[prec][r]) == seen.end())

@@ -50,7 +50,26 @@ struct SolveArgs {
   // global-state variants: per-solve scratch at gstate + block * gstride
   double *gstate;
   int64_t gstride;          // doubles per solve
+  // Step chunks (ramp mode, register kernel).  The launch runs
+  // n_chunks * group_n workgroups: workgroup i runs chunk i / group_n of the
+  // group's solve i % group_n, steps [c * chunk_steps, (c + 1) * chunk_steps)
+  // (the last chunk to n_steps).  Between chunks a solve's lane state lives at
+  // cstate + (i % group_n) * cstride floats, and cflag[solve] counts its
+  // finished chunks (release/acquire at agent scope; the last chunk resets it
+  // to 0, so every launch starts from zeros).  n_chunks <= 1: no chunks.
+  int32_t n_chunks, chunk_steps, group_n, pad3_;
+  float *cstate;
+  int64_t cstride;          // floats per solve
+  int32_t *cflag;
 };
+
+// Floats of chunk state per solve of a register-kernel workgroup of B
+// threads and NPT node slots per thread: (u hi, u lo, v) x 3 per slot and
+// lane, then the step size and the residual.
+constexpr int64_t chunk_stride(int B, int NPT) { return ((int64_t)NPT * 9 * B + 4 + 63) & ~63; }
+// Spin limit of a chunk waiting for its predecessor (s_sleep 127 per try,
+// ~3.4 us): ~14 s, far beyond any chunk; a solve that runs out is reported NaN.
+constexpr int kChunkSpinLimit = 1 << 22;
 
 // A kernel variant: threads per solve, free-node slots per thread and the
 // placement flags.
@@ -93,6 +112,9 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 hipError_t launch_solve(const Variant &v, const SolveArgs &a, int n_blocks,
                         size_t lds, hipStream_t s);
+// Workgroups of variant v (with `lds` dynamic LDS bytes) resident per CU;
+// 0 when v has no step-chunk support.
+int chunk_occupancy(const Variant &v, size_t lds);
 hipError_t lds_probe(double *bytes_per_s);
 hipError_t hbm_probe(int64_t bytes, double *bytes_per_s);
 hipError_t launch_broadcast_F(const double *F, double *out, int n, hipStream_t s);

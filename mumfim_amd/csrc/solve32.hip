@@ -476,9 +476,18 @@ void solve32r(SolveArgs a) {
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
-  const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  // step chunk of this workgroup (ramp mode): chunk `chunk` of the group's
+  // solve `unit`
+  int unit = blockIdx.x, chunk = 0;
+  const bool chunked = a.n_chunks > 1;
+  if (chunked) {
+    chunk = unit / a.group_n;
+    unit -= chunk * a.group_n;
+  }
+  const bool last_chunk = !chunked || chunk == a.n_chunks - 1;
+  const int sid = a.order ? a.order[unit] : unit;
   if (__builtin_amdgcn_groupstaticsize() != 0) {  // the LDS addressing below needs base 0
-    if (tid == 0) {
+    if (tid == 0 && last_chunk) {
       for (int i = 0; i < 6; ++i) a.sigma[6 * (int64_t)sid + i] = __builtin_nan("");
       a.status[sid] = MF_STATUS_NAN;
     }
@@ -632,6 +641,34 @@ void solve32r(SolveArgs a) {
     for (int o = 32; o > 0; o >>= 1) D = max(D, __shfl_xor(D, o, kWave));
     Ds[sl] = __builtin_amdgcn_readfirstlane(D);
   }
+  // a later chunk continues from the state its predecessor saved
+  float *cs = chunked ? a.cstate + (int64_t)unit * a.cstride : nullptr;
+  float h_saved = 0.0f, res_saved = 0.0f;
+  if (chunk > 0) {
+    if (tid == 0) {
+      bool ok = false;
+      for (int spin = 0; spin < kChunkSpinLimit; ++spin) {
+        if (__hip_atomic_load(a.cflag + sid, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= chunk) {
+          ok = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(127);
+      }
+      if (!ok) cs[NPT * 9 * B + 1] = __builtin_nanf("");  // reported as NaN at the end
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every lane sees the saved state
+#pragma unroll
+    for (int sl = 0; sl < NPT; ++sl)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        uh[sl][d] = cs[(sl * 9 + d) * B + tid];
+        ul[sl][d] = cs[(sl * 9 + 3 + d) * B + tid];
+        vr[sl][d] = cs[(sl * 9 + 6 + d) * B + tid];
+      }
+    h_saved = cs[NPT * 9 * B];
+    res_saved = cs[NPT * 9 * B + 1];
+  }
   __syncthreads();
 
   const float inv_c = (float)(1.0 / R.c_sound);
@@ -744,13 +781,13 @@ void solve32r(SolveArgs a) {
   // applied as soon as a_k and h_{k+1} are known (v_{-1/2} = 0, h_{-1} = 0),
   // so no acceleration is carried across the barrier.
   const float crit_half = (float)(a.crit * 0.5);
-  float h_prev = 0.0f;
+  float h_prev = h_saved;
   float fint[NPT][3];
   // acceleration of a slot's node: linear forces already carry 1/m
   auto accel = [&](int sl, int d) __attribute__((always_inline)) {
     return LIN ? fint[sl][d] : fint[sl][d] * imr[sl];
   };
-  {
+  if (chunk == 0) {  // a later chunk's velocities already hold this kick
     double resid_local = 0.0;
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl)
@@ -769,12 +806,13 @@ void solve32r(SolveArgs a) {
     h_prev = h;
   }
 
-  int step = 0;
-  double residual = 10.0;
+  int step = chunked ? chunk * a.chunk_steps : 0;
+  const int step_end = last_chunk ? a.n_steps : step + a.chunk_steps;
+  double residual = chunk > 0 ? (double)res_saved : 10.0;
   int status = MF_STATUS_OK;
   const int freq = a.freq;
   // one step whose positions go to buffer NXT (a compile-time constant);
-  // returns true when the solve is done
+  // returns true when the solve (or its chunk) is done
   auto one_step = [&](auto NXTC) __attribute__((always_inline)) -> bool {
     constexpr int nxt = decltype(NXTC)::value;
     const float h2 = 2.0f * h_prev;
@@ -838,7 +876,7 @@ void solve32r(SolveArgs a) {
       }
     h_prev = h;
     ++step;
-    if (ramp) return step >= a.n_steps;
+    if (ramp) return step >= step_end;
     if (!(residual > a.tol)) return true;
     if (step >= a.max_steps) {
       status = MF_STATUS_STEP_CAP;
@@ -847,10 +885,33 @@ void solve32r(SolveArgs a) {
     return false;
   };
   // the positions start in buffer 0; steps alternate buffers 1, 0, 1, ...
+  // (every step writes all positions of its buffer before reading it, so a
+  // later chunk may start on either buffer)
   for (;;) {
     if (one_step(std::integral_constant<int, 1>{})) break;
     if (one_step(std::integral_constant<int, 0>{})) break;
   }
+  if (!last_chunk) {
+    // save the lane state, publish it, raise the solve's chunk count
+#pragma unroll
+    for (int sl = 0; sl < NPT; ++sl)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cs[(sl * 9 + d) * B + tid] = uh[sl][d];
+        cs[(sl * 9 + 3 + d) * B + tid] = ul[sl][d];
+        cs[(sl * 9 + 6 + d) * B + tid] = vr[sl][d];
+      }
+    if (tid == 0) {
+      cs[NPT * 9 * B] = h_prev;
+      cs[NPT * 9 * B + 1] = (float)residual;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0)
+      __hip_atomic_store(a.cflag + sid, chunk + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  if (chunked && tid == 0) a.cflag[sid] = 0;  // the next launch starts from zero
   if (residual != residual) status = MF_STATUS_NAN;
 
   // finalize: low parts into the spare buffer, boundary gathers in FP32 with
@@ -1829,6 +1890,39 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
 }
 
 }  // namespace kern
+
+int chunk_occupancy(const Variant &v, size_t lds) {
+  if (v.prec != MF_PREC_FP32C || !(v.flags & kEllInRegs)) return 0;
+  int n = 0;
+#define MF_REG_OCC(B, N, RMODE, FLAG)                                          \
+  if (v.npt == N && v.block == B && (v.flags & (kLinearOnly | kMix2)) == FLAG) { \
+    auto k = kern::solve32r<B, N, RMODE>;                                      \
+    if (lds > 64 * 1024)                                                       \
+      (void)hipFuncSetAttribute((const void *)k,                               \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)kMaxLds);                                 \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, lds) != hipSuccess) n = 0; \
+    return n;                                                                  \
+  }
+#define MF_REG_OCC2(B, N)                                                      \
+  MF_REG_OCC(B, N, kern::RM_LIN, kLinearOnly)                                  \
+  MF_REG_OCC(B, N, kern::RM_MIX2, kMix2)
+  MF_REG_OCC(256, 1, kern::RM_TABLE, 0)
+  MF_REG_OCC(512, 1, kern::RM_TABLE, 0)
+  MF_REG_OCC2(256, 1)
+  MF_REG_OCC2(512, 1)
+  MF_REG_OCC2(640, 1)
+  MF_REG_OCC2(768, 1)
+  MF_REG_OCC2(1024, 1)
+  MF_REG_OCC(768, 2, kern::RM_LIN, kLinearOnly)
+  MF_REG_OCC(256, 2, kern::RM_LIN, kLinearOnly)
+  MF_REG_OCC(512, 3, kern::RM_LIN, kLinearOnly)
+  MF_REG_OCC(512, 3, kern::RM_MIX2, kMix2)
+  MF_REG_OCC(512, 4, kern::RM_LIN, kLinearOnly)
+#undef MF_REG_OCC2
+#undef MF_REG_OCC
+  return 0;
+}
 
 hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
                   size_t lds, hipStream_t s) {

@@ -115,3 +115,73 @@ def test_graph_replay_matches_eager(prec, mode):
         assert np.array_equal(xg, xe) and np.array_equal(ug, ue) and np.array_equal(Vg, Ve)
     ms, nl = graph.last_solve_ms()
     assert ms > 0 and nl >= 1
+
+
+@pytest.mark.parametrize("chunks,steps,nl", [(2, 1001, 0.0), (3, 1000, 0.0), (4, 999, 0.5)])
+def test_step_chunks_bitwise(chunks, steps, nl):
+    """Solves split into step chunks (state saved to HBM between chunks, each
+    chunk waiting for its predecessor's flag) give bitwise the unchunked
+    stress, statuses and committed state, also through a graph replay and
+    on a second run (the chunk counters reset themselves)."""
+    say(f"case: step chunks={chunks} steps={steps} nonlinear={nl}")
+    nets = [mf.generate_network(seed=700 + i, n_nodes=n, nonlinear_fraction=nl)
+            for i, n in enumerate([500] * 40 + [300, 650, 900])]
+    R = len(nets)
+    F = strain_F(30, R, 0.05)
+    kw = dict(mode=mf.MF_MODE_RAMP, n_steps=steps, precision=mf.MF_PREC_FP32C,
+              reaction_policy=mf.MF_REACT_PER_FIBER, fp32_kernel=mf.MF_FP32_KERNEL_NODE)
+    g1 = mf.BatchedFiberRVEAnalysisExplicit(nets, mf.default_params(chunks=1, **kw))
+    gc = mf.BatchedFiberRVEAnalysisExplicit(nets, mf.default_params(chunks=chunks, **kw))
+    _, p1 = g1.plan()
+    _, pc = gc.plan()
+    assert not np.any(p1[:, 2] >> 24)
+    reg = (pc[:, 2] & 4) != 0  # register-kernel RVEs run in chunks
+    assert reg.sum() >= 40 and np.all((pc[reg, 2] >> 24) == chunks), pc
+    for update in (False, True, False):
+        s1, st1 = g1.run(F, update_coords=update, return_status=True)
+        sc, stc = gc.run(F, update_coords=update, return_status=True)
+        assert (st1 == 0).all() and np.array_equal(stc, st1)
+        assert np.array_equal(sc, s1)
+    x1, u1, _ = g1.get_state()
+    xc, uc, _ = gc.get_state()
+    assert np.array_equal(xc, x1) and np.array_equal(uc, u1)
+    # graph replays of the chunked launch
+    F_dev = mf.DeviceArray.from_host(F.reshape(R, 9))
+    sig = mf.DeviceArray((R, 6))
+    g1.run_device(F_dev, sig)
+    g1.synchronize()
+    ref = sig.numpy()
+    gc.set_graph(True)
+    for _ in range(3):
+        gc.run_device(F_dev, sig)
+        gc.synchronize()
+        assert np.array_equal(sig.numpy(), ref)
+        gc.run(F, update_coords=False)  # eager work between replays
+    gc.set_graph(False)
+
+
+def test_planner_chunks_the_500_node_batch():
+    """On the 4096 x 500-node batch the planner splits the register-kernel
+    group into step chunks on its own; the stress is bitwise the unchunked
+    one."""
+    R = 4096
+    say(f"case: planner step chunks, {R} x 500 nodes")
+    gen = mf.default_gen_params(seed=1, n_nodes=500)
+    F = strain_F(2, R, 0.05)
+    kw = dict(mode=mf.MF_MODE_RAMP, n_steps=1000, precision=mf.MF_PREC_FP32C)
+    ga = mf.BatchedFiberRVEAnalysisExplicit(params=mf.default_params(**kw), generated=(gen, R))
+    _, plan = ga.plan()
+    nch = plan[:, 2] >> 24
+    say(f"  chunks per solve: {np.unique(nch, return_counts=True)}")
+    assert (nch > 1).sum() >= 4000
+    sa, sta = ga.run(F, update_coords=False, return_status=True)
+    ms_chunked = ga.last_solve_ms()[0]
+    ga.close()
+    g1 = mf.BatchedFiberRVEAnalysisExplicit(params=mf.default_params(chunks=1, **kw),
+                                            generated=(gen, R))
+    s1 = g1.run(F, update_coords=False)
+    ms_plain = g1.last_solve_ms()[0]
+    g1.close()
+    say(f"  kernel ms: chunked {ms_chunked:.2f}, one piece {ms_plain:.2f}")
+    assert (sta == 0).all()
+    assert np.array_equal(sa, s1)
