@@ -106,6 +106,12 @@ typedef struct mf_network {
 #define MF_REACT_PER_FIBER 1
 #define MF_TANGENT_FORWARD 0     /* 6 solves, (sigma(h) - sigma)/h          */
 #define MF_TANGENT_CENTRAL 1     /* 12 solves, (sigma(+h) - sigma(-h))/(2h) */
+/* Ramp mode, tangent_precision MF_PREC_FP32C: the h -> 0 limit of both
+ * schemes, the derivative of each perturbed solve along its perturbation
+ * (dF = E_i F, E_i the unit Voigt strain of direction i) integrated with the
+ * solve itself (tangent-linear), 6 solves per RVE and no difference of
+ * nearly equal stresses; fd_step is not used. */
+#define MF_TANGENT_LINEAR 2
 
 typedef struct mf_solver_params {
   int32_t mode;                 /* MF_MODE_*                                  */

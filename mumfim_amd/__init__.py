@@ -21,7 +21,7 @@ MF_OK = 0
 MF_MODE_RELAX, MF_MODE_RAMP = 0, 1
 MF_PREC_FP64, MF_PREC_FP32C = 0, 1
 MF_REACT_BATCHED_REFERENCE, MF_REACT_PER_FIBER = 0, 1
-MF_TANGENT_FORWARD, MF_TANGENT_CENTRAL = 0, 1
+MF_TANGENT_FORWARD, MF_TANGENT_CENTRAL, MF_TANGENT_LINEAR = 0, 1, 2
 MF_REACTION_LINEAR, MF_REACTION_NONLINEAR = 0, 1
 MF_STATUS_OK, MF_STATUS_NAN, MF_STATUS_STEP_CAP, MF_STATUS_FALLBACK = 0, 1, 2, 3
 MF_STATUS_ENERGY = 4

@@ -27,6 +27,7 @@ SOURCES = {
     "aux_kernels.hip": ["-ffp-contract=off"],
     "solve64.hip": ["-ffp-contract=off"],
     "solve32.hip": [],
+    "solve32t.hip": [],
     "mlp.hip": [],
 }
 

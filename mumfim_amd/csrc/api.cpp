@@ -102,6 +102,20 @@ void matmul3(const double A[9], const double B[9], double C[9]) {
     }
 }
 
+// d P_i / d h at h = 0 of the perturbation above: the unit Voigt strain of
+// direction i (engineering shear: 1/2 off the diagonal).
+void perturbation_rate(int i, double E[9]) {
+  for (int k = 0; k < 9; ++k) E[k] = 0;
+  switch (i) {
+    case 0: E[0] = 1; break;
+    case 1: E[4] = 1; break;
+    case 2: E[8] = 1; break;
+    case 3: E[5] = E[7] = 0.5; break;
+    case 4: E[2] = E[6] = 0.5; break;
+    default: E[1] = E[3] = 0.5; break;
+  }
+}
+
 bool params_valid(const mf_solver_params &p) {
   return (p.mode == MF_MODE_RELAX || p.mode == MF_MODE_RAMP) &&
          (p.precision == MF_PREC_FP64 || p.precision == MF_PREC_FP32C) &&
@@ -109,8 +123,9 @@ bool params_valid(const mf_solver_params &p) {
           p.tangent_precision == MF_PREC_FP32C) &&
          (p.reaction_policy == MF_REACT_BATCHED_REFERENCE ||
           p.reaction_policy == MF_REACT_PER_FIBER) &&
-         (p.tangent_scheme == MF_TANGENT_FORWARD ||
-          p.tangent_scheme == MF_TANGENT_CENTRAL) &&
+         (p.tangent_scheme == MF_TANGENT_FORWARD || p.tangent_scheme == MF_TANGENT_CENTRAL ||
+          (p.tangent_scheme == MF_TANGENT_LINEAR && p.mode == MF_MODE_RAMP &&
+           p.tangent_precision == MF_PREC_FP32C)) &&
          p.dt_update_freq >= 1 && p.max_steps >= 1 &&
          (p.mode != MF_MODE_RAMP || p.n_steps >= 1) && p.fd_step > 0 &&
          p.fd_step < 0.25 && p.crit_time_scale_factor > 0 &&
@@ -190,6 +205,10 @@ struct mf_batch {
   // plan: per RVE a variant index per precision
   std::vector<int> plan[2];
   std::vector<size_t> lds[2];
+  // tangent-linear variant per RVE (-1: the RVE fits none) and its LDS bytes
+  std::vector<int> plan_tl;
+  std::vector<size_t> lds_tl;
+  DevBuf<double> solve_G, dsigma;
   int no_reg_variants = 0;  // planner restricted to the LDS/global-list kernels
   std::map<TuneKey, int> tuned;  // autotuned variant per bucket
   int plan_version = 0;          // bumped whenever the launch tables change
@@ -396,6 +415,31 @@ int make_plan(mf_batch *b, int prec) {
   return MF_OK;
 }
 
+// Tangent-linear variant of every RVE: the fewest threads, then the fewest
+// node slots per thread, that hold its free nodes and fit the LDS.
+void make_plan_tl(mf_batch *b) {
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  b->plan_tl.assign(b->R, -1);
+  b->lds_tl.assign(b->R, 0);
+  for (int r = 0; r < b->R; ++r) {
+    const PackedRVE &p = b->host[r];
+    long best_key = 0;
+    for (int i = 0; i < nv; ++i) {
+      const Variant &v = vs[i];
+      if (!(v.flags & kTangentLinear) || (long)v.block * v.npt < p.Nf) continue;
+      const size_t L = lds_bytes(v, p.N, p.Nb, p.M, p.ell_free_entries);
+      if (L > kMaxLds) continue;
+      const long key = (long)v.npt * 100000 + v.block;
+      if (b->plan_tl[r] < 0 || key < best_key) {
+        b->plan_tl[r] = i;
+        b->lds_tl[r] = L;
+        best_key = key;
+      }
+    }
+  }
+}
+
 int upload_batch(mf_batch *b) {
   const int R = b->R;
   std::vector<RveDev> rd(R);
@@ -567,6 +611,7 @@ int create_from_networks(std::vector<Network> &nets, const mf_solver_params *par
     int e = make_plan(b.get(), prec);
     if (e != MF_OK) return e;
   }
+  make_plan_tl(b.get());
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     set_error("no HIP device available");
@@ -622,6 +667,10 @@ struct SolveSet {
   std::vector<int32_t> rve, flags;
   std::vector<double> F;  // 9 per solve
   int prec = MF_PREC_FP64;
+  // tangent-linear solves: direction of each perturbation (9 per solve);
+  // d sigma lands in b->dsigma
+  bool tl = false;
+  std::vector<double> G;
 };
 
 int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
@@ -637,7 +686,30 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   MF_HIP(b->status.ensure(n));
   MF_HIP(b->steps.ensure(n));
   const int prec = S.prec;
-  const std::vector<int32_t> order = launch_order(b, prec, n, [&](int i) { return S.rve[i]; });
+  if (S.tl) {
+    for (int i = 0; i < n; ++i)
+      if (b->plan_tl[S.rve[i]] < 0) {
+        set_error("RVE " + std::to_string(S.rve[i]) + " (" + std::to_string(b->host[S.rve[i]].N) +
+                  " nodes) is too large for the tangent-linear kernel; use the central scheme");
+        return MF_ERR_UNSUPPORTED;
+      }
+    MF_HIP(b->solve_G.ensure(9 * (size_t)n));
+    MF_HIP(b->dsigma.ensure(6 * (size_t)n));
+    MF_HIP(hipMemcpyAsync(b->solve_G.p, S.G.data(), sizeof(double) * 9 * n,
+                          hipMemcpyHostToDevice, s));
+  }
+  const std::vector<int> &plan = S.tl ? b->plan_tl : b->plan[prec];
+  const std::vector<size_t> &plds = S.tl ? b->lds_tl : b->lds[prec];
+  std::vector<int32_t> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  if (S.tl)  // grouped by variant, largest RVEs first
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const int vx = plan[S.rve[x]], vy = plan[S.rve[y]];
+      if (vx != vy) return vx > vy;
+      return b->host[S.rve[x]].M > b->host[S.rve[y]].M;
+    });
+  else
+    order = launch_order(b, prec, n, [&](int i) { return S.rve[i]; });
   MF_HIP(hipMemcpyAsync(b->solve_rve.p, S.rve.data(), sizeof(int32_t) * n,
                         hipMemcpyHostToDevice, s));
   MF_HIP(hipMemcpyAsync(b->solve_flags.p, S.flags.data(), sizeof(int32_t) * n,
@@ -683,18 +755,22 @@ int launch_set(mf_batch *b, const SolveSet &S, const double *x_in,
   a.energy_eps = b->p.energy_check_eps;
   a.crit = b->p.crit_time_scale_factor;
   a.tol = b->p.tolerance;
+  if (S.tl) {
+    a.solve_G = b->solve_G.p;
+    a.dsigma = b->dsigma.p;
+  }
   std::vector<std::pair<int, int>> groups;
   std::vector<size_t> glds;
   std::vector<int64_t> ggs;
   int nv = 0;
   const Variant *vs = variants(&nv);
   for (int i = 0; i < n;) {
-    int v = b->plan[prec][S.rve[order[i]]];
+    int v = plan[S.rve[order[i]]];
     int j = i;
     size_t L = 0;
     int maxN = 0;
-    while (j < n && b->plan[prec][S.rve[order[j]]] == v) {
-      L = std::max(L, b->lds[prec][S.rve[order[j]]]);
+    while (j < n && plan[S.rve[order[j]]] == v) {
+      L = std::max(L, plds[S.rve[order[j]]]);
       maxN = std::max(maxN, b->host[S.rve[order[j]]].N);
       ++j;
     }
@@ -1160,6 +1236,44 @@ int tangent_impl(mf_batch *b, double *Cdev, std::vector<int32_t> *status_out) {
                           hipMemcpyDeviceToHost, s));
     MF_HIP(hipStreamSynchronize(s));
     b->lastF_on_device = false;
+  }
+  if (b->p.tangent_scheme == MF_TANGENT_LINEAR) {
+    // 6 tangent-linear solves per RVE from the state before the last update
+    // run: F_last with its derivative along E_i F_last
+    SolveSet S;
+    S.prec = MF_PREC_FP32C;
+    S.tl = true;
+    for (int r = 0; r < R; ++r) {
+      const double *F0 = b->lastF.data() + 9 * (size_t)r;
+      for (int i = 0; i < 6; ++i) {
+        double E[9], G[9];
+        perturbation_rate(i, E);
+        matmul3(E, F0, G);
+        S.rve.push_back(r);
+        S.flags.push_back(0);
+        S.F.insert(S.F.end(), F0, F0 + 9);
+        S.G.insert(S.G.end(), G, G + 9);
+      }
+    }
+    const int p = b->cur ^ 1;
+    int e = launch_set(b, S, b->xs[p].p, b->us[p].p, nullptr, nullptr);
+    if (e != MF_OK) return e;
+    hipError_t he = launch_tl_assemble(b->dsigma.p, Cdev, R, s);
+    if (he != hipSuccess) {
+      set_error(hipGetErrorString(he));
+      return MF_ERR_HIP;
+    }
+    if (status_out) {
+      std::vector<int32_t> st(S.rve.size());
+      MF_HIP(hipMemcpyAsync(st.data(), b->status.p, sizeof(int32_t) * st.size(),
+                            hipMemcpyDeviceToHost, s));
+      MF_HIP(hipStreamSynchronize(s));
+      status_out->assign(R, MF_STATUS_OK);
+      for (size_t k = 0; k < st.size(); ++k)
+        if ((*status_out)[S.rve[k]] == MF_STATUS_OK) (*status_out)[S.rve[k]] = st[k];
+    }
+    MF_HIP(hipStreamSynchronize(s));
+    return MF_OK;
   }
   const int per = central ? 13 : 7;
   SolveSet S;
@@ -1721,12 +1835,27 @@ int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
     return MF_ERR_HIP;
   }
   const bool central = b->p.tangent_scheme == MF_TANGENT_CENTRAL;
-  const int per = central ? 13 : 7;
+  const bool tl = b->p.tangent_scheme == MF_TANGENT_LINEAR;
+  const int per = tl ? 6 : central ? 13 : 7;
   const double h = b->p.fd_step;
   SolveSet S;
   S.prec = b->p.precision;
+  S.tl = tl;
   for (int r = 0; r < R; ++r) {
     const double *F0 = F + 9 * (nF == 1 ? 0 : r);
+    if (tl) {
+      // solve 0 of each RVE commits the state; all six carry the stress
+      for (int i = 0; i < 6; ++i) {
+        double E[9], G[9];
+        perturbation_rate(i, E);
+        matmul3(E, F0, G);
+        S.rve.push_back(r);
+        S.flags.push_back(i == 0 ? (kWriteX | kWriteU) : 0);
+        S.F.insert(S.F.end(), F0, F0 + 9);
+        S.G.insert(S.G.end(), G, G + 9);
+      }
+      continue;
+    }
     S.rve.push_back(r);
     S.flags.push_back(kWriteX | kWriteU);
     S.F.insert(S.F.end(), F0, F0 + 9);
@@ -1747,8 +1876,9 @@ int mf_batch_run_with_tangent(mf_batch *b, const double *F, int32_t nF,
   b->have_energy = false;  // this run's solves track no energies
   DevBuf<double> Cd;
   MF_HIP(Cd.alloc(36 * (size_t)R));
-  he = launch_fd_all(b->sigma.p, Cd.p, R,
-                     central ? MF_TANGENT_CENTRAL : MF_TANGENT_FORWARD, h, s);
+  he = tl ? launch_tl_assemble(b->dsigma.p, Cd.p, R, s)
+          : launch_fd_all(b->sigma.p, Cd.p, R, central ? MF_TANGENT_CENTRAL : MF_TANGENT_FORWARD,
+                          h, s);
   if (he != hipSuccess) {
     set_error(hipGetErrorString(he));
     return MF_ERR_HIP;

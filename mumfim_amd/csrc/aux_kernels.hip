@@ -77,6 +77,14 @@ __global__ void fd_kernel(const double *sigma, double *C, int n, int scheme,
   C[(int64_t)r * 36 + j * 6 + i] = v;
 }
 
+// Tangent-linear columns: solve 6 r + i holds d sigma along direction i.
+__global__ void tl_assemble_kernel(const double *dsigma, double *C, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * 36) return;
+  const int r = idx / 36, j = (idx % 36) / 6, i = idx % 6;
+  C[(int64_t)r * 36 + j * 6 + i] = dsigma[((int64_t)r * 6 + i) * 6 + j];
+}
+
 // One FD column with separate base/perturbed arrays (relax-mode sequence,
 // BatchedFiberRVEAnalysisExplicit.h:394-402).
 __global__ void fd_column_kernel(const double *sig1, const double *sig0,
@@ -189,6 +197,10 @@ static const Variant kVariants[] = {
     {MF_PREC_FP64, 1024, 2, kFiberOnce},
     // FP64 with the state in global memory: RVEs no other variant holds
     {MF_PREC_FP64, 1024, 1, kGlobalState},
+    // FP32c tangent-linear: (threads, free nodes per thread)
+#define MF_TL_ENTRY(B, N) {MF_PREC_FP32C, B, N, kTangentLinear},
+    MF_TL_LIST(MF_TL_ENTRY)
+#undef MF_TL_ENTRY
 };
 
 const Variant *variants(int *count) {
@@ -206,6 +218,12 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
     return sizeof(double) * 6 * Np + sizeof(double) * (((size_t)M + 1) & ~size_t(1)) +
            sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp +
            ((sizeof(ReactDev) * kMaxReactions + 15) & ~size_t(15));
+  }
+  if (v.flags & kTangentLinear) {
+    // displacements, their derivatives, reference coordinates; scratch;
+    // boundary state (FP64) and its derivative (FP32); packed table
+    return 48 * Np + sizeof(double) * NW * kern::kRedStride + sizeof(double) * 6 * Nbp +
+           16 * Nbp + sizeof(float) * 4 * 3 * kMaxReactions;
   }
   if (v.flags & kGlobalState) {
     // scratch, boundary state, reactions (positions etc. in global memory)
@@ -248,10 +266,13 @@ hipError_t launch_solve64(const Variant &v, const SolveArgs &a, int n_blocks,
                           size_t lds, hipStream_t s);
 hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
                           size_t lds, hipStream_t s);
+hipError_t launch_solve32t(const Variant &v, const SolveArgs &a, int n_blocks,
+                           size_t lds, hipStream_t s);
 
 hipError_t launch_solve(const Variant &v, const SolveArgs &a, int n_blocks,
                         size_t lds, hipStream_t s) {
   if (n_blocks <= 0) return hipSuccess;
+  if (v.flags & kTangentLinear) return launch_solve32t(v, a, n_blocks, lds, s);
   return v.prec == MF_PREC_FP64 ? launch_solve64(v, a, n_blocks, lds, s)
                                 : launch_solve32(v, a, n_blocks, lds, s);
 }
@@ -334,6 +355,13 @@ hipError_t launch_fd_all(const double *sigma, double *C, int n, int scheme,
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(kern::fd_kernel, dim3((n * 36 + 255) / 256), dim3(256), 0, s,
                      sigma, C, n, scheme, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_tl_assemble(const double *dsigma, double *C, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern::tl_assemble_kernel, dim3((n * 36 + 255) / 256), dim3(256), 0, s,
+                     dsigma, C, n);
   return hipGetLastError();
 }
 

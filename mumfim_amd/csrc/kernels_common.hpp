@@ -117,6 +117,18 @@ __device__ inline void block_min_sum(double &a, double &b, double *red) {
 
 __device__ inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
+// whole 16-byte loads of records in LDS through generic pointers
+typedef float ld16_f4 __attribute__((ext_vector_type(4), aligned(16)));
+typedef int ld16_i4 __attribute__((ext_vector_type(4), aligned(16)));
+__device__ inline float4 ld16(const float4 *p) {
+  const ld16_f4 v = *reinterpret_cast<const ld16_f4 *>(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int4 ld16(const int4 *p) {
+  const ld16_i4 v = *reinterpret_cast<const ld16_i4 *>(p);
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+
 // ---- reaction laws, FP64 reference formulas (FiberReactions.cc:5-71) ------
 __device__ inline double nl_force(double alpha, double B, double lr,
                                   double g) {
@@ -248,6 +260,46 @@ __device__ inline float react32_mix2(const ReactDev &nl, float ea_half_lin, bool
   return is_nl ? ((hi || lo) ? te : tt) : ea_half_lin * il * e2;
 }
 
+// t and dt/de2 of react32_tab (tangent-linear kernel).  Linear: EA/2 il;
+// toe: d/de2 alpha il expm1(B/2 e2) = B/2 (alpha il + t); beyond a
+// transition t = (K (lam - L0) + b) il / lam with lam = sqrt(1 + e2), whose
+// derivative is il (K L0 - b) / (2 lam^3).
+__device__ inline float react32_tab_d(const float4 *T, unsigned rid, float e2, float il,
+                                      float &dt) {
+  const float4 a = T[kReactTabStride * rid];
+  const float4 b = T[kReactTabStride * rid + 1], c = T[kReactTabStride * rid + 2];
+  const float tl = a.x * il * e2;
+  const bool hi = e2 > b.x, lo = e2 < c.x;
+  const float x = a.z * fminf(fmaxf(e2, c.x), b.x);
+  const float em = fabsf(x) <= kToePolyMax ? expm1_poly(x) : __expf(x) - 1.0f;
+  const float tt = a.y * il * em;
+  const float lam2 = 1.0f + e2;
+  const float ilam = __frsqrt_rn(lam2);
+  const float lam = lam2 * ilam;
+  const float K = hi ? b.z : c.z, L0 = hi ? b.y : c.y, bb = hi ? b.w : c.w;
+  const float te = (K * (lam - L0) + bb) * il * ilam;
+  const float dte = 0.5f * (K * L0 - bb) * il * (ilam * ilam * ilam);
+  const bool ext = hi || lo;
+  dt = a.w != 0.0f ? (ext ? dte : a.z * (a.y * il + tt)) : a.x * il;
+  return a.w != 0.0f ? (ext ? te : tt) : tl;
+}
+
+// t and dt/de2 of react32_mix2.
+__device__ inline float react32_mix2_d(const ReactDev &nl, float ea_half_lin, bool is_nl,
+                                       float e2, float il, float &dt) {
+  const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
+  const float tt = il * toe_poly(nl, fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1));
+  const float lam2 = 1.0f + e2;
+  const float ilam = __frsqrt_rn(lam2);
+  const float lam = lam2 * ilam;
+  const float K = hi ? nl.Kt : nl.Kc, L0 = hi ? nl.lt_f : nl.lc_f, bb = hi ? nl.bt : nl.bc;
+  const float te = (K * (lam - L0) + bb) * il * ilam;
+  const float dte = 0.5f * (K * L0 - bb) * il * (ilam * ilam * ilam);
+  const bool ext = hi || lo;
+  dt = is_nl ? (ext ? dte : nl.B_half * (nl.alpha * il + tt)) : ea_half_lin * il;
+  return is_nl ? (ext ? te : tt) : ea_half_lin * il * e2;
+}
+
 // The reaction table qualifies for react32_mix2 (uniform over the
 // workgroup); nl_rid and ea_half_lin describe it.
 __device__ inline bool mix2_table(const ReactDev *q, int n, int &nl_rid, float &ea_half_lin) {
@@ -275,6 +327,9 @@ struct Lin64 {
 
 }  // namespace kern
 }  // namespace mf
+
+// Tangent-linear FP32c variants: (threads, free-node slots per thread).
+#define MF_TL_LIST(X) X(256, 1) X(512, 1) X(1024, 1) X(1024, 2) X(1024, 3)
 
 // Fiber-streamed FP32c variants (large RVEs): (threads, fibers per thread,
 // free nodes per thread, reaction mode flag kLinearOnly|kMix2).  The widest

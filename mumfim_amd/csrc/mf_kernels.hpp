@@ -61,6 +61,10 @@ struct SolveArgs {
   float *cstate;
   int64_t cstride;          // floats per solve
   int32_t *cflag;
+  // tangent-linear variants: direction of each solve's perturbation of F
+  // (9 per solve) and the stress derivative along it (6 per solve)
+  const double *solve_G;
+  double *dsigma;
 };
 
 // Floats of chunk state per solve of a register-kernel workgroup of B
@@ -84,6 +88,7 @@ enum : int32_t {
   kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
   kGlobalState = 8192, // FP64 state in global memory: any RVE size
   kFiberOnce = 16384,  // FP64, t = f/l once per fiber, nodes gather by reference
+  kTangentLinear = 32768,  // FP32c solve with its derivative along solve_G
 };
 // Stride (float4 entries) of the two position buffers of the register
 // kernel: buffer 1 starts 32 KiB into LDS, a compile-time offset the LDS
@@ -125,6 +130,8 @@ hipError_t launch_fd(const double *sig_a, const double *sig_b, double *C,
                      int stride_b, hipStream_t s);
 hipError_t launch_fd_all(const double *sigma, double *C, int n_rve, int scheme,
                          double h, hipStream_t s);
+// C of n_rve RVEs from tangent-linear solves 6 r + i (d sigma along i).
+hipError_t launch_tl_assemble(const double *dsigma, double *C, int n_rve, hipStream_t s);
 hipError_t launch_orientation(const RveDev *rves, const double *x,
                               const int32_t *fib, const double *normal,
                               int n_normal, double *omega, int n_rve,

@@ -21,15 +21,6 @@ typedef __attribute__((address_space(3))) const unsigned char lds_uchar;
 typedef float v4f __attribute__((ext_vector_type(4), aligned(16)));
 typedef __attribute__((address_space(3))) const v4f lds_v4f;
 typedef int v4i __attribute__((ext_vector_type(4), aligned(16)));
-// whole 16-byte loads of records in LDS through generic pointers
-__device__ inline float4 ld16(const float4 *p) {
-  const v4f v = *reinterpret_cast<const v4f *>(p);
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ inline int4 ld16(const int4 *p) {
-  const v4i v = *reinterpret_cast<const v4i *>(p);
-  return make_int4(v.x, v.y, v.z, v.w);
-}
 
 // ============================================================================
 // FP32 + compensated kernel

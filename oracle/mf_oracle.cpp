@@ -394,6 +394,197 @@ int solve_rve(ORve &R, const mf_solver_params &p, const double F[9],
   return step;
 }
 
+// ---- tangent-linear (ramp mode) -----------------------------------------
+// d f / d l of a fiber's reaction (FiberReactions.cc:5-71): linear
+// EA (3 lambda^2 - 1) / (2 l0); toe law alpha/l0 ((1 + lambda^2 B) e^{B g} - 1);
+// beyond a transition the slope at the transition.
+double fiber_dforce(const ORve &R, int e, double l0, double l) {
+  double E = R.E, A = R.A;
+  if (R.per_fiber) {
+    const OReaction &q = R.reactions[R.fib_react[e]];
+    E = q.E;
+    A = q.area();
+    if (q.type != MF_REACTION_LINEAR) {
+      const double lr = l / l0, alpha = E * A / q.B;
+      double lam = lr;
+      if (lr > q.lt) lam = q.lt;
+      else if (lr < q.lc) lam = q.lc;
+      return nl_deriv(alpha, q.B, lam, l0, 0.5 * (lam * lam - 1));
+    }
+  }
+  const double lr = l / l0;
+  return E * A * (3 * lr * lr - 1) / (2 * l0);
+}
+
+// Fiber lengths and their derivatives along dx; returns the minimum length
+// (first index on ties, as element_lengths) and its derivative in *dmin.
+double element_lengths_tl(const ORve &R, const std::vector<double> &c,
+                          const std::vector<double> &dc, std::vector<double> &out,
+                          std::vector<double> &dout, double *dmin) {
+  double lmin = 1e300, dl = 0;
+  for (int i = 0; i < R.M; ++i) {
+    const int n1 = R.conn[2 * i], n2 = R.conn[2 * i + 1];
+    double s = 0, ds = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double d = c[3 * n2 + k] - c[3 * n1 + k];
+      s += d * d;
+      ds += d * (dc[3 * n2 + k] - dc[3 * n1 + k]);
+    }
+    out[i] = sqrt(s);
+    dout[i] = ds / out[i];
+    if (out[i] < lmin) {
+      lmin = out[i];
+      dl = dout[i];
+    }
+  }
+  *dmin = dl;
+  return lmin;
+}
+
+// Internal forces and their derivatives over all fibers, then the total
+// force (and derivative) of the free vertices [0, nv).
+void get_forces_tl(ORve &R, int nv, double eta, const std::vector<double> &dx,
+                   const std::vector<double> &dl, const std::vector<double> &dv,
+                   std::vector<double> &df_int, std::vector<double> &df) {
+  std::fill(R.f_int.begin(), R.f_int.end(), 0.0);
+  std::fill(df_int.begin(), df_int.end(), 0.0);
+  for (int i = 0; i < R.M; ++i) {
+    const double l = R.l[i], l0 = R.l0[i];
+    const double frc = fiber_force(R, i, l0, l);
+    const double t = frc / l;
+    const double dt = fiber_dforce(R, i, l0, l) * dl[i] / l - t * dl[i] / l;
+    const int n1 = R.conn[2 * i], n2 = R.conn[2 * i + 1];
+    for (int k = 0; k < 3; ++k) {
+      const double d = R.x[3 * n2 + k] - R.x[3 * n1 + k];
+      const double dd = dx[3 * n2 + k] - dx[3 * n1 + k];
+      const double e = d * t, de = dd * t + d * dt;
+      R.f_int[3 * n1 + k] -= e;
+      R.f_int[3 * n2 + k] += e;
+      df_int[3 * n1 + k] -= de;
+      df_int[3 * n2 + k] += de;
+    }
+  }
+  for (int i = 0; i < nv; ++i) {
+    const double mass = R.mass[i] * eta;
+    for (int j = 0; j < 3; ++j) {
+      R.f[3 * i + j] = -R.f_int[3 * i + j] - mass * R.v[3 * i + j];
+      df[3 * i + j] = -df_int[3 * i + j] - mass * dv[3 * i + j];
+    }
+  }
+}
+
+// Tangent-linear of one ramp-mode trial solve from the current state: the
+// derivative of the solve's stress along a perturbation of its deformation
+// gradient, dF = G (here E_i F_last).  Every quantity the solve computes is
+// differentiated in the same order: the boundary ramp (d x_b = alpha G x_b),
+// the half kicks and drifts, the fiber forces (d(f/l) from d f/d l), the
+// critical step (through the shortest fiber, refreshed with it) and the
+// boundary stress sum.  The state is left untouched.
+void solve_rve_tl(ORve &R, const mf_solver_params &p, const double F[9],
+                  const double G[9], double sigma[6], double dsigma[6]) {
+  const int N = R.N, Nf = R.N - R.Nb;
+  const std::vector<double> u_copy = R.u, x_copy = R.x;
+  std::vector<double> g(3 * (size_t)N), dg(3 * (size_t)N);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < 3; ++k) {
+      g[3 * i + k] = (F[3 * k + 0] - (k == 0 ? 1 : 0)) * R.x[3 * i] +
+                     (F[3 * k + 1] - (k == 1 ? 1 : 0)) * R.x[3 * i + 1] +
+                     (F[3 * k + 2] - (k == 2 ? 1 : 0)) * R.x[3 * i + 2];
+      dg[3 * i + k] = G[3 * k + 0] * R.x[3 * i] + G[3 * k + 1] * R.x[3 * i + 1] +
+                      G[3 * k + 2] * R.x[3 * i + 2];
+    }
+  std::fill(R.v.begin(), R.v.end(), 0.0);
+  std::vector<double> dx(3 * (size_t)N, 0.0), dv(3 * (size_t)N, 0.0), df(3 * (size_t)N, 0.0),
+      df_int(3 * (size_t)N, 0.0), dl(R.M, 0.0);
+  double ddt_crit = 0;
+  for (int i = 0; i < 3 * N; ++i) R.x[i] = R.X[i] + R.u[i];
+  std::vector<double> xb0(3 * (size_t)R.Nb);
+  for (int b = 0; b < R.Nb; ++b)
+    for (int k = 0; k < 3; ++k) xb0[3 * b + k] = R.x[3 * (Nf + b) + k];
+  double dt_crit = element_lengths_tl(R, R.x, dx, R.l, dl, &ddt_crit) / R.c;
+  ddt_crit /= R.c;
+  get_forces_tl(R, Nf, p.visc_damp_coeff, dx, dl, dv, df_int, df);
+  const double crit = p.crit_time_scale_factor;
+  int step = 0;
+  do {
+    const double dt = crit * dt_crit * 0.5, ddt = crit * ddt_crit * 0.5;
+    for (int i = 0; i < Nf; ++i) {
+      const double mass = R.mass[i];
+      for (int j = 0; j < 3; ++j) {
+        const int q = 3 * i + j;
+        const double v_local = R.v[q] + dt * R.f[q] / mass;
+        const double dv_local = dv[q] + ddt * R.f[q] / mass + dt * df[q] / mass;
+        R.v[q] = v_local;
+        dv[q] = dv_local;
+        R.x[q] = R.x[q] + 2 * dt * v_local;
+        dx[q] = dx[q] + 2 * (ddt * v_local + dt * dv_local);
+      }
+    }
+    const double alpha = ramp_alpha(step + 1, p.n_steps, p.amplitude);
+    for (int b = 0; b < R.Nb; ++b)
+      for (int k = 0; k < 3; ++k) {
+        const int q = 3 * (Nf + b) + k;
+        R.x[q] = xb0[3 * b + k] + alpha * g[q];
+        dx[q] = alpha * dg[q];
+      }
+    double dlmin = 0;
+    const double lmin = element_lengths_tl(R, R.x, dx, R.l, dl, &dlmin);
+    if (step % p.dt_update_freq == 0) {
+      dt_crit = lmin / R.c;
+      ddt_crit = dlmin / R.c;
+    }
+    get_forces_tl(R, Nf, p.visc_damp_coeff, dx, dl, dv, df_int, df);
+    for (int i = 0; i < Nf; ++i) {
+      const double mass = R.mass[i];
+      for (int j = 0; j < 3; ++j) {
+        const int q = 3 * i + j;
+        R.v[q] += dt * R.f[q] / mass;
+        dv[q] += ddt * R.f[q] / mass + dt * df[q] / mass;
+      }
+    }
+    ++step;
+  } while (step < p.n_steps);
+  get_forces_tl(R, N, p.visc_damp_coeff, dx, dl, dv, df_int, df);
+  double s[6] = {0, 0, 0, 0, 0, 0}, ds[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < R.Nb; ++b) {
+    const int vert = R.fixed[b];
+    const double *f = &R.f_int[3 * vert], *c = &R.x[3 * vert];
+    const double *dfi = &df_int[3 * vert], *dc = &dx[3 * vert];
+    s[0] += c[0] * f[0];
+    s[1] += c[1] * f[1];
+    s[2] += c[2] * f[2];
+    s[3] += 0.5 * (c[1] * f[2] + c[2] * f[1]);
+    s[4] += 0.5 * (c[0] * f[2] + c[2] * f[0]);
+    s[5] += 0.5 * (c[0] * f[1] + c[1] * f[0]);
+    ds[0] += dc[0] * f[0] + c[0] * dfi[0];
+    ds[1] += dc[1] * f[1] + c[1] * dfi[1];
+    ds[2] += dc[2] * f[2] + c[2] * dfi[2];
+    ds[3] += 0.5 * (dc[1] * f[2] + c[1] * dfi[2] + dc[2] * f[1] + c[2] * dfi[1]);
+    ds[4] += 0.5 * (dc[0] * f[2] + c[0] * dfi[2] + dc[2] * f[0] + c[2] * dfi[0]);
+    ds[5] += 0.5 * (dc[0] * f[1] + c[0] * dfi[1] + dc[1] * f[0] + c[1] * dfi[0]);
+  }
+  for (int i = 0; i < 6; ++i) {
+    sigma[i] = s[i] * (R.scale / R.V);
+    dsigma[i] = ds[i] * (R.scale / R.V);
+  }
+  R.u = u_copy;
+  R.x = x_copy;
+}
+
+// d P_i / d h at h = 0 of the perturbation below: the unit Voigt strain of
+// direction i with engineering shear (1 on the diagonal, 1/2 off it).
+void perturbation_rate(int i, double E[9]) {
+  for (int k = 0; k < 9; ++k) E[k] = 0;
+  switch (i) {
+    case 0: E[0] = 1; break;
+    case 1: E[4] = 1; break;
+    case 2: E[8] = 1; break;
+    case 3: E[5] = E[7] = 0.5; break;
+    case 4: E[2] = E[6] = 0.5; break;
+    default: E[1] = E[3] = 0.5; break;
+  }
+}
+
 // Almansi perturbation (BatchedFiberRVEAnalysisExplicit.h:340-389).
 void perturbation(int i, double h, double Fp[9]) {
   double D1 = sqrt(1.0 / (1 - 2 * h));
@@ -531,6 +722,28 @@ int oracle_tangent(oracle_batch *b, double *C) {
   for (int r = 0; r < n; ++r) {
     ORve &R = b->rves[r];
     double sp[6], sm[6];
+    if (b->p.tangent_scheme == MF_TANGENT_LINEAR) {
+      // ramp mode: derivative of each trial solve from the state before the
+      // last update run along G = E_i F_last
+      std::vector<double> xs = R.x, us = R.u;
+      for (int i = 0; i < 6; ++i) {
+        double E[9], G[9];
+        perturbation_rate(i, E);
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c) {
+            double s1 = 0;
+            for (int k = 0; k < 3; ++k) s1 += E[3 * a + k] * R.F_last[3 * k + c];
+            G[3 * a + c] = s1;
+          }
+        R.x = R.x_prev;
+        R.u = R.u_prev;
+        solve_rve_tl(R, b->p, R.F_last, G, sp, sm);
+        for (int j = 0; j < 6; ++j) C[36 * (size_t)r + 6 * j + i] = sm[j];
+      }
+      R.x = xs;
+      R.u = us;
+      continue;
+    }
     for (int i = 0; i < 6; ++i) {
       double Fp[9], Fm[9];
       perturbation(i, h, Fp);

@@ -437,3 +437,50 @@ def test_orientation_2d_non_unit_normal_known_answer():
     for c, w in want.items():
         om = o.orientation(normal=[0.0, 0.0, c])[0]
         assert np.allclose(om, w, rtol=0, atol=1e-14), (c, om)
+
+
+def test_tangent_linear_affine_patch_closed_form():
+    """Oracle tangent-linear (ramp mode) on the all-boundary patch after two
+    update runs: the derivative of the closed-form affine stress along
+    P_i(h) F2 at h = 0, taken here by a central difference of the closed
+    form at h = 1e-5 (its truncation, O(h^2), is ~1e-9 relative: it falls
+    100-fold from h = 1e-4)."""
+    net = _affine_patch()
+    X = net.coords
+    F1 = np.array([[1.02, 0.01, 0.0], [0.0, 0.99, 0.005], [0.003, 0.0, 1.01]])
+    F2 = np.array([[0.995, 0.0, 0.012], [0.004, 1.015, 0.0], [0.0, -0.008, 1.0]])
+    p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=200, tangent_scheme=mf.MF_TANGENT_LINEAR)
+    o = Oracle([net], p)
+    o.run(F1)
+    o.run(F2)
+    C = o.computeMaterialStiffness()[0]
+    V = np.linalg.det(F1) * np.linalg.det(F2)
+    x1 = X @ F1.T
+    h = 1e-5
+    Cc = np.zeros((6, 6))
+    for i in range(6):
+        sp = _fiber_virial(net, x1 @ (_perturbation(i, h) @ F2).T) / V
+        sm = _fiber_virial(net, x1 @ (_perturbation(i, -h) @ F2).T) / V
+        Cc[:, i] = (sp - sm) / (2 * h)
+    scale = np.abs(Cc).max()
+    assert np.abs(C - Cc).max() <= 1e-8 * scale, np.abs(C - Cc).max() / scale
+
+
+@pytest.mark.parametrize("nl", [0.0, 0.5])
+def test_tangent_linear_matches_central_difference(nl):
+    """On a generated 300-node network (linear, or half the fibers on the toe
+    law) the oracle's tangent-linear equals its own central difference of
+    whole ramp solves at h = 1e-6 (FP64) to the difference's accuracy:
+    every term of the solve (ramp, kicks, forces, the refreshed critical
+    step, the boundary sum) is differentiated."""
+    net = mf.generate_network(seed=41, n_nodes=300, E=1e8, radius=5e-5, nonlinear_fraction=nl)
+    F = np.array([[1.03, 0.01, -0.004], [0.01, 0.985, 0.002], [-0.004, 0.002, 1.012]])
+    kw = dict(mode=mf.MF_MODE_RAMP, n_steps=1500, reaction_policy=mf.MF_REACT_PER_FIBER)
+    ot = Oracle([net], mf.default_params(tangent_scheme=mf.MF_TANGENT_LINEAR, **kw))
+    ot.run(F)
+    Ct = ot.computeMaterialStiffness()[0]
+    oc = Oracle([net], mf.default_params(tangent_scheme=mf.MF_TANGENT_CENTRAL, fd_step=1e-6, **kw))
+    oc.run(F)
+    Cc = oc.computeMaterialStiffness()[0]
+    err = np.abs(Ct - Cc).max() / np.abs(Cc).max()
+    assert err <= 1e-7, err
