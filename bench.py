@@ -495,11 +495,29 @@ def extra_tangent1k2k():
     _, Cm, st = g.run_with_tangent(F, return_status=True)
     wall = time.perf_counter() - t
     g.close()
+    # FP32 tangent-linear: 6 solves per RVE, each with its derivative along
+    # one perturbation direction, no difference of nearly equal stresses
+    pt = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER,
+                           precision=mf.MF_PREC_FP32C, tangent_precision=mf.MF_PREC_FP32C,
+                           tangent_scheme=mf.MF_TANGENT_LINEAR)
+    gt = mf.BatchedFiberRVEAnalysisExplicit(params=pt, generated=(gen, R), n_nodes=sizes)
+    t = time.perf_counter()
+    _, Ct, stt = gt.run_with_tangent(F, return_status=True)
+    wt = time.perf_counter() - t
+    gt.close()
+    e = rel_err(Ct.reshape(R, 36), Cm.reshape(R, 36))
     return {"workload": f"{R} RVEs, nodes U[1000,2000], 50% toe-law fibers, base + 12 central "
                         f"perturbations (h=1e-6) = {13 * R} solves of {S} ramp steps",
             "fp64_tangents_per_s": round(R / wall, 1),
             "fp64_solves_per_s": round(13 * R / wall, 1), "ms_per_batch": round(1e3 * wall, 1),
-            "status_all_ok": bool((st == 0).all()), "tangent_finite": bool(np.isfinite(Cm).all())}
+            "status_all_ok": bool((st == 0).all() and (stt == 0).all()),
+            "tangent_finite": bool(np.isfinite(Cm).all() and np.isfinite(Ct).all()),
+            "fp32_tangent_linear": {
+                "tangents_per_s": round(R / wt, 1), "ms_per_batch": round(1e3 * wt, 1),
+                "speedup_vs_fp64_central": round(wall / wt, 3),
+                "C_rel_err_vs_fp64_central": {"median": float(np.median(e)),
+                                              "p99": float(np.percentile(e, 99)),
+                                              "max": float(e.max())}}}
 
 
 def main():
