@@ -520,6 +520,38 @@ def extra_tangent1k2k():
                                               "max": float(e.max())}}}
 
 
+def bench_tangentlinear(args):
+    """FP32 tangent-linear stress + tangent of the 4096 RVEs of 1000-2000
+    nodes (half the fibers on the toe law, 5000 ramp steps): 6 solves per
+    RVE, each carrying its derivative along one perturbation direction.
+    Trial batches from the reference state (the state is restored after each
+    timed call)."""
+    import mumfim_amd as mf
+    R, S = 4096, 5000
+    u = mf.uniform(4, 11, R)
+    sizes = (1000 + np.floor(u * 1001)).astype(np.int32)
+    F = strain_F(4, R, 0.05)
+    gen = mf.default_gen_params(seed=200001, nonlinear_fraction=0.5)
+    pt = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, reaction_policy=mf.MF_REACT_PER_FIBER,
+                           precision=mf.MF_PREC_FP32C, tangent_precision=mf.MF_PREC_FP32C,
+                           tangent_scheme=mf.MF_TANGENT_LINEAR)
+    times = []
+    for k in range(args.warmup + args.steps):
+        g = mf.BatchedFiberRVEAnalysisExplicit(params=pt, generated=(gen, R), n_nodes=sizes)
+        t = time.perf_counter()
+        _, Ct, st = g.run_with_tangent(F, return_status=True)
+        dt = time.perf_counter() - t
+        g.close()
+        if k >= args.warmup:
+            times.append(dt)
+        log(f"[tangentlinear] batch {k}: {dt:.3f}s, status ok {bool((st == 0).all())}")
+    wall = float(np.mean(times))
+    print(json.dumps({"config": "tangentlinear", "value": round(R / wall, 2),
+                      "unit": "tangents/s (stress + 6x6 tangent per RVE)",
+                      "ms_per_batch": round(1e3 * wall, 1), "steps": args.steps,
+                      "dtype": "f32"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -534,10 +566,12 @@ def main():
                     help="skip the mixed-size and tangent workloads of the default run")
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
-    ap.add_argument("--config", choices=["batch500", "mixed16k", "tangent1k2k", "surrogate"], default="batch500",
+    ap.add_argument("--config", choices=["batch500", "mixed16k", "tangent1k2k", "tangentlinear",
+                                         "surrogate"], default="batch500",
                     help="batch500 (default): 4096 x 500 nodes; mixed16k: 16384 RVEs of 200-4000 "
                          "nodes, mixed reactions, 10000 steps; tangent1k2k: 4096 RVEs of "
-                         "1000-2000 nodes, base + 12 perturbed solves per RVE; surrogate: learned "
+                         "1000-2000 nodes, base + 12 perturbed solves per RVE; tangentlinear: the "
+                         "same RVEs, FP32 tangent-linear only (6 solves per RVE); surrogate: learned "
                          "surrogate trained on 1M solves, inference on 1M points")
     ap.add_argument("--surrogate-networks", type=int, default=65536,
                     help="surrogate: networks in the training data (16 strains each)")
@@ -559,6 +593,8 @@ def main():
         return bench_tangent1k2k(args)
     if args.config == "surrogate":
         return bench_surrogate(args)
+    if args.config == "tangentlinear":
+        return bench_tangentlinear(args)
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])

@@ -307,6 +307,9 @@ bool mix2_ok(const PackedRVE &p) {
 // energy diagnostics) under the batch parameters.
 bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE &p,
                   bool lin) {
+  // tangent-linear kernels need a direction and a derivative buffer per
+  // solve: only tangent-linear solve sets (make_plan_tl) run them
+  if (v.flags & kTangentLinear) return false;
   if (v.flags & kGlobalState)  // FP64, any size; also the FP32c fallback
     return !(prec == MF_PREC_FP64 && b->p.energy_diagnostics) &&
            (!b->p.block_size || v.block == b->p.block_size) &&

@@ -72,24 +72,46 @@ __device__ inline double ramp_alpha(int k, int S, int amplitude) {
   return xi;
 }
 
-// Ramp factor of the FP32 kernels' step loop, whose boundary positions are
-// rounded to FP32: xi = k * (1/S) is within an ulp of k / S and saves an FP64
-// division per step.  The final stress uses ramp_alpha.
-__device__ inline double ramp_alpha_step(int k, double inv_S, int amplitude) {
-  const double xi = (double)k * inv_S;
-  if (amplitude == MF_AMP_SMOOTHSTEP) return xi * xi * xi * (10 - 15 * xi + 6 * xi * xi);
-  return xi;
+
+// Boundary record of the FP32 kernels (48 bytes of LDS, 16-byte aligned):
+// the start displacement u0 and the ramp increment g = (F - I) x_b in FP64
+// (u0x u0y u0z gx gy gz), read as three 16-byte pairs.  The step's position
+// u0 + alpha g is formed in FP64 and rounded once to FP32: a boundary
+// displacement is ~0.03 while a boundary fiber stretches by ~5e-3, so an ulp
+// of the position is ~4e-7 of the stretch, and an FP32 evaluation (three
+// roundings) measured 3-4x the median stress error of the rounded FP64 one.
+__device__ inline void bnd_set(double *rec, const double (&u0)[3], const double (&g)[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    rec[k] = u0[k];
+    rec[3 + k] = g[k];
+  }
 }
 
-// Boundary displacement u_b = u0 + alpha g of the FP32 kernels from a
-// boundary node's 48-byte LDS record (u0x u0y u0z gx gy gz, 16-byte aligned),
-// read as three 16-byte pairs.
+// u_b = u0 + alpha g in FP64.
 __device__ inline void bnd_disp(const double *rec, double alpha, double (&ub)[3]) {
   const double2 *p = reinterpret_cast<const double2 *>(rec);
   const double2 q0 = p[0], q1 = p[1], q2 = p[2];
   ub[0] = fma(alpha, q1.y, q0.x);
   ub[1] = fma(alpha, q2.x, q0.y);
   ub[2] = fma(alpha, q2.y, q1.x);
+}
+
+// The step's boundary displacement, rounded once to FP32.
+__device__ inline void bnd_step(const double *rec, double alpha, float (&ub)[3]) {
+  double u[3];
+  bnd_disp(rec, alpha, u);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) ub[k] = (float)u[k];
+}
+
+// Ramp factor of the FP32 kernels' step loop: xi = k * (1/S) is within an
+// ulp of k / S and saves an FP64 division per step (the positions it moves
+// are rounded to FP32 anyway); the final stress uses ramp_alpha.
+__device__ inline double ramp_alpha_step(int k, double inv_S, int amplitude) {
+  const double xi = (double)k * inv_S;
+  if (amplitude == MF_AMP_SMOOTHSTEP) return xi * xi * xi * (10 - 15 * xi + 6 * xi * xi);
+  return xi;
 }
 
 // min of a (slot 0) and sum of b (slot 1) in one pass.

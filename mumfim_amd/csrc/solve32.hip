@@ -119,10 +119,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        bst[6 * b + k] = u[k];
-        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
-      }
+      for (int k = 0; k < 3; ++k)
+        if (!ramp) inc[k] = 0.0;
+      bnd_set(bst + 6 * b, u, inc);
     }
   };
   const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
@@ -272,9 +271,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
-        double ub[3];
-        bnd_disp(bst + 6 * b, alpha, ub);
-        Pn[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+        float ub[3];
+        bnd_step(bst + 6 * b, alpha, ub);
+        Pn[Nf + b] = make_float4(ub[0], ub[1], ub[2], 0.0f);
       }
     }
     __syncthreads();
@@ -542,10 +541,9 @@ void solve32r(SolveArgs a) {
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        bst[6 * b + k] = u[k];
-        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
-      }
+      for (int k = 0; k < 3; ++k)
+        if (!ramp) inc[k] = 0.0;
+      bnd_set(bst + 6 * b, u, inc);
     }
   };
   for (int rb = tid; rb < N; rb += B) {
@@ -829,9 +827,9 @@ void solve32r(SolveArgs a) {
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
-        double ub[3];
-        bnd_disp(bst + 6 * b, alpha, ub);
-        Pn[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+        float ub[3];
+        bnd_step(bst + 6 * b, alpha, ub);
+        Pn[Nf + b] = make_float4(ub[0], ub[1], ub[2], 0.0f);
       }
     }
     __syncthreads();
@@ -1081,10 +1079,9 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        bst[6 * b + k] = u[k];
-        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
-      }
+      for (int k = 0; k < 3; ++k)
+        if (!ramp) inc[k] = 0.0;
+      bnd_set(bst + 6 * b, u, inc);
     }
   }
   for (int i = tid; i < nslot; i += B) fsl[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1320,9 +1317,9 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
-        double ub[3];
-        bnd_disp(bst + 6 * b, alpha, ub);
-        pos[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+        float ub[3];
+        bnd_step(bst + 6 * b, alpha, ub);
+        pos[Nf + b] = make_float4(ub[0], ub[1], ub[2], 0.0f);
       }
     }
     __syncthreads();  // positions published; every slot read of the last step done
@@ -1552,10 +1549,9 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        bst[6 * b + k] = u[k];
-        bst[6 * b + 3 + k] = ramp ? inc[k] : 0.0;
-      }
+      for (int k = 0; k < 3; ++k)
+        if (!ramp) inc[k] = 0.0;
+      bnd_set(bst + 6 * b, u, inc);
     }
   }
   for (int i = tid; i < nff; i += B) ffr[i] = F3{0.0f, 0.0f, 0.0f};
@@ -1735,9 +1731,9 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
       for (int b = tid; b < Nb; b += B) {
-        double ub[3];
-        bnd_disp(bst + 6 * b, alpha, ub);
-        pos[Nf + b] = F3{(float)ub[0], (float)ub[1], (float)ub[2]};
+        float ub[3];
+        bnd_step(bst + 6 * b, alpha, ub);
+        pos[Nf + b] = F3{ub[0], ub[1], ub[2]};
       }
     }
     __syncthreads();  // positions published; every force read of the last step done

@@ -11,8 +11,8 @@
 // solves is taken, so it holds at any strain scale.
 //
 // LDS: displacement hi parts and their derivatives (one buffer each, two
-// barriers per step), fixed-point reference coordinates, boundary state
-// (u_b0, g in FP64; dg in FP32); incidence lists stream from HBM.
+// barriers per step), fixed-point reference coordinates, boundary records
+// (bnd_set, FP64) and dg in FP32; incidence lists stream from HBM.
 #include "kernels_common.hpp"
 
 namespace mf {
@@ -86,6 +86,13 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int sid = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  if (!a.solve_G || !a.dsigma) {  // launched outside a tangent-linear set
+    if (tid == 0) {
+      for (int i = 0; i < 6; ++i) a.sigma[6 * (int64_t)sid + i] = __builtin_nan("");
+      a.status[sid] = MF_STATUS_NAN;
+    }
+    return;
+  }
   const int rv = a.solve_rve[sid];
   const RveDev R = a.rves[rv];
   const int N = R.N, Nf = R.Nf, Nb = R.Nb, M = R.M;
@@ -146,15 +153,16 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
                       (int)rint(a.X[3 * g + 2] * R.xq_scale), 0);
     if (r >= Nf) {
       const int b = r - Nf;
+      double inc[3];
       float dg[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        bst[6 * b + k] = u[k];
-        bst[6 * b + 3 + k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
-                             (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
-                             (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
+        inc[k] = (F[3 * k + 0] - (k == 0 ? 1.0 : 0.0)) * xp[0] +
+                 (F[3 * k + 1] - (k == 1 ? 1.0 : 0.0)) * xp[1] +
+                 (F[3 * k + 2] - (k == 2 ? 1.0 : 0.0)) * xp[2];
         dg[k] = (float)(G[3 * k + 0] * xp[0] + G[3 * k + 1] * xp[1] + G[3 * k + 2] * xp[2]);
       }
+      bnd_set(bst + 6 * b, u, inc);
       dbst[b] = make_float4(dg[0], dg[1], dg[2], 0.0f);
     }
   };
@@ -331,9 +339,9 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
       const float af = (float)alpha;
       for (int b = tid; b < Nb; b += B) {
-        double ub[3];
-        bnd_disp(bst + 6 * b, alpha, ub);
-        pos[Nf + b] = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
+        float ub[3];
+        bnd_step(bst + 6 * b, alpha, ub);
+        pos[Nf + b] = make_float4(ub[0], ub[1], ub[2], 0.0f);
         const float4 g = dbst[b];
         dpos[Nf + b] = make_float4(af * g.x, af * g.y, af * g.z, 0.0f);
       }
