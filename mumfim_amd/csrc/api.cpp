@@ -47,18 +47,18 @@ void make_react_dev(const Reaction &q, ReactDev &d) {
     const double alpha = EA / q.B;
     d.alpha = (float)alpha;
     d.B_half = (float)(0.5 * q.B);
-    auto ext = [&](double lam, float &K, float &b) {
+    auto ext = [&](double lam, float &K, float &C) {
       // f(lam) beyond a transition: m*(lam - lt)*l0 + b with m*l0 and b
-      // independent of l0 (FiberReactions.cc:16-62)
+      // independent of l0 (FiberReactions.cc:16-62); t = f/l = il (K - C/lam)
       double g = 0.5 * (lam * lam - 1);
-      K = (float)(alpha * ((1 + lam * lam * q.B) * std::exp(q.B * g) - 1));
-      b = (float)(alpha * lam * (std::exp(q.B * g) - 1));
+      const double k = alpha * ((1 + lam * lam * q.B) * std::exp(q.B * g) - 1);
+      const double b = alpha * lam * (std::exp(q.B * g) - 1);
+      K = (float)k;
+      C = (float)(k * lam - b);
     };
-    ext(q.lt, d.Kt, d.bt);
-    ext(q.lc, d.Kc, d.bc);
-    d.lt_f = (float)q.lt;
+    ext(q.lt, d.Kt, d.Ct);
+    ext(q.lc, d.Kc, d.Cc);
     d.lt2m1 = (float)(q.lt * q.lt - 1.0);
-    d.lc_f = (float)q.lc;
     d.lc2m1 = (float)(q.lc * q.lc - 1.0);
     double c = alpha;
     for (int k = 1; k <= 7; ++k) {

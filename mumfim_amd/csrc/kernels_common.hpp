@@ -186,27 +186,20 @@ __device__ inline double react64(const ReactDev &q, double l0, double l) {
 // ---- reaction laws, FP32 algebraic forms --------------------------------
 // t = f/l as a function of e2 = lambda^2 - 1 = (l^2 - l0^2)/l0^2, which the
 // FP32 kernel forms without cancellation from the displacement difference.
-__device__ inline float react32_t(const ReactDev &q, float e2, float il) {
-  if (q.type == MF_REACTION_LINEAR) return q.EA_half * il * e2;
-  if (e2 > q.lt2m1) {
-    const float lam2 = 1.0f + e2;
-    const float ilam = __frsqrt_rn(lam2);
-    const float lam = lam2 * ilam;
-    return (q.Kt * (lam - q.lt_f) + q.bt) * il * ilam;
-  } else if (e2 < q.lc2m1) {
-    const float lam2 = 1.0f + e2;
-    const float ilam = __frsqrt_rn(lam2);
-    const float lam = lam2 * ilam;
-    return (q.Kc * (lam - q.lc_f) + q.bc) * il * ilam;
-  }
-  return q.alpha * il * expm1f(q.B_half * e2);
+// Linear: t = EA/2 il e2.  Toe: t = alpha il expm1(B/2 e2).  Beyond a
+// transition: f = K (lam - L0) + b, so t = il (K - C / lam) with
+// C = K L0 - b folded on the host and 1/lam = rsqrt(1 + e2): one rsqrt and
+// two operations, as accurate in FP32 as the direct form (both lose the
+// same absolute eps K near the transition).
+__device__ inline float ext_law(float K, float C, float il, float ilam) {
+  return il * fmaf(-C, ilam, K);
 }
 
-// Branch-free form of react32_t over a reaction table repacked in LDS as
+// Branch-free toe/extension law over a reaction table repacked in LDS as
 // three float4 per reaction:
 //   [0] EA/2, alpha = EA/B, B/2, 1 if nonlinear else 0
-//   [1] lt^2 - 1, lt, Kt, bt        (tension extension, lambda > lt)
-//   [2] lc^2 - 1, lc, Kc, bc        (compression extension, lambda < lc)
+//   [1] lt^2 - 1, Kt, Ct, 0         (tension extension, lambda > lt)
+//   [2] lc^2 - 1, Kc, Cc, 0         (compression extension, lambda < lc)
 // In the toe region x = B/2 (lambda^2 - 1) lies in [B/2 (lc^2-1), B/2 (lt^2-1)];
 // clamping e2 to that range keeps the polynomial's argument inside it, and
 // the selected branch ignores the clamped value outside the region.
@@ -216,8 +209,8 @@ constexpr int kReactTabStride = 3;    // float4 per reaction
 __device__ inline void pack_react_table(const ReactDev &q, float4 *t) {
   const bool nl = q.type == MF_REACTION_NONLINEAR;
   t[0] = make_float4(q.EA_half, nl ? q.alpha : 0.0f, nl ? q.B_half : 0.0f, nl ? 1.0f : 0.0f);
-  t[1] = make_float4(nl ? q.lt2m1 : 0.0f, q.lt_f, q.Kt, q.bt);
-  t[2] = make_float4(nl ? q.lc2m1 : 0.0f, q.lc_f, q.Kc, q.bc);
+  t[1] = make_float4(nl ? q.lt2m1 : 0.0f, q.Kt, q.Ct, 0.0f);
+  t[2] = make_float4(nl ? q.lc2m1 : 0.0f, q.Kc, q.Cc, 0.0f);
 }
 
 // expm1(x) for |x| <= kToePolyMax: Taylor series to x^7 (truncation below
@@ -258,34 +251,33 @@ __device__ inline float react32_tab(const float4 *T, unsigned rid, float e2, flo
   const float x = a.z * fminf(fmaxf(e2, c.x), b.x);
   const float em = fabsf(x) <= kToePolyMax ? expm1_poly(x) : __expf(x) - 1.0f;
   const float tt = a.y * il * em;
-  const float lam2 = 1.0f + e2;
-  const float ilam = __frsqrt_rn(lam2);
-  const float lam = lam2 * ilam;
-  const float K = hi ? b.z : c.z, L0 = hi ? b.y : c.y, bb = hi ? b.w : c.w;
-  const float te = (K * (lam - L0) + bb) * il * ilam;
+  const float te = ext_law(hi ? b.y : c.y, hi ? b.z : c.z, il, __frsqrt_rn(1.0f + e2));
   const float tn = (hi || lo) ? te : tt;
   return a.w != 0.0f ? tn : tl;
 }
 
+// t of a nonlinear reaction whose toe range lies within the polynomial's
+// bound, branch-free: the toe polynomial and the extension are both
+// evaluated and one is selected.  Outside the toe range the polynomial's
+// value is dropped, so its argument needs no clamp.
+__device__ inline float toe_ext_t(const ReactDev &nl, float e2, float il) {
+  const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
+  const float tt = il * toe_poly(nl, e2);
+  const float te = ext_law(hi ? nl.Kt : nl.Kc, hi ? nl.Ct : nl.Cc, il, __frsqrt_rn(1.0f + e2));
+  return (hi || lo) ? te : tt;
+}
+
 // One linear (EA/2 = ea_half_lin) and one nonlinear reaction (nl) with a
-// toe range within the polynomial's bound: t for a fiber of either law,
-// branch-free (see react32_tab).
+// toe range within the polynomial's bound: t for a fiber of either law.
 __device__ inline float react32_mix2(const ReactDev &nl, float ea_half_lin, bool is_nl,
                                      float e2, float il) {
-  const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
-  const float tt = il * toe_poly(nl, fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1));
-  const float lam2 = 1.0f + e2;
-  const float ilam = __frsqrt_rn(lam2);
-  const float lam = lam2 * ilam;
-  const float K = hi ? nl.Kt : nl.Kc, L0 = hi ? nl.lt_f : nl.lc_f, bb = hi ? nl.bt : nl.bc;
-  const float te = (K * (lam - L0) + bb) * il * ilam;
-  return is_nl ? ((hi || lo) ? te : tt) : ea_half_lin * il * e2;
+  return is_nl ? toe_ext_t(nl, e2, il) : ea_half_lin * il * e2;
 }
 
 // t and dt/de2 of react32_tab (tangent-linear kernel).  Linear: EA/2 il;
 // toe: d/de2 alpha il expm1(B/2 e2) = B/2 (alpha il + t); beyond a
-// transition t = (K (lam - L0) + b) il / lam with lam = sqrt(1 + e2), whose
-// derivative is il (K L0 - b) / (2 lam^3).
+// transition t = il (K - C / lam) with lam = sqrt(1 + e2), whose derivative
+// is il C / (2 lam^3).
 __device__ inline float react32_tab_d(const float4 *T, unsigned rid, float e2, float il,
                                       float &dt) {
   const float4 a = T[kReactTabStride * rid];
@@ -295,12 +287,10 @@ __device__ inline float react32_tab_d(const float4 *T, unsigned rid, float e2, f
   const float x = a.z * fminf(fmaxf(e2, c.x), b.x);
   const float em = fabsf(x) <= kToePolyMax ? expm1_poly(x) : __expf(x) - 1.0f;
   const float tt = a.y * il * em;
-  const float lam2 = 1.0f + e2;
-  const float ilam = __frsqrt_rn(lam2);
-  const float lam = lam2 * ilam;
-  const float K = hi ? b.z : c.z, L0 = hi ? b.y : c.y, bb = hi ? b.w : c.w;
-  const float te = (K * (lam - L0) + bb) * il * ilam;
-  const float dte = 0.5f * (K * L0 - bb) * il * (ilam * ilam * ilam);
+  const float ilam = __frsqrt_rn(1.0f + e2);
+  const float C = hi ? b.z : c.z;
+  const float te = ext_law(hi ? b.y : c.y, C, il, ilam);
+  const float dte = 0.5f * C * il * (ilam * ilam * ilam);
   const bool ext = hi || lo;
   dt = a.w != 0.0f ? (ext ? dte : a.z * (a.y * il + tt)) : a.x * il;
   return a.w != 0.0f ? (ext ? te : tt) : tl;
@@ -310,13 +300,11 @@ __device__ inline float react32_tab_d(const float4 *T, unsigned rid, float e2, f
 __device__ inline float react32_mix2_d(const ReactDev &nl, float ea_half_lin, bool is_nl,
                                        float e2, float il, float &dt) {
   const bool hi = e2 > nl.lt2m1, lo = e2 < nl.lc2m1;
-  const float tt = il * toe_poly(nl, fminf(fmaxf(e2, nl.lc2m1), nl.lt2m1));
-  const float lam2 = 1.0f + e2;
-  const float ilam = __frsqrt_rn(lam2);
-  const float lam = lam2 * ilam;
-  const float K = hi ? nl.Kt : nl.Kc, L0 = hi ? nl.lt_f : nl.lc_f, bb = hi ? nl.bt : nl.bc;
-  const float te = (K * (lam - L0) + bb) * il * ilam;
-  const float dte = 0.5f * (K * L0 - bb) * il * (ilam * ilam * ilam);
+  const float tt = il * toe_poly(nl, e2);
+  const float ilam = __frsqrt_rn(1.0f + e2);
+  const float C = hi ? nl.Ct : nl.Cc;
+  const float te = ext_law(hi ? nl.Kt : nl.Kc, C, il, ilam);
+  const float dte = 0.5f * C * il * (ilam * ilam * ilam);
   const bool ext = hi || lo;
   dt = is_nl ? (ext ? dte : nl.B_half * (nl.alpha * il + tt)) : ea_half_lin * il;
   return is_nl ? (ext ? te : tt) : ea_half_lin * il * e2;

@@ -149,10 +149,11 @@ struct ReactDev {
   float alpha;     // nonlinear alpha = E*A/B
   float B_half;    // B/2
   float lt2m1;     // lambda_t^2 - 1
-  float lt_f;
-  float Kt, bt;    // tension extension: f = Kt*(lam - lt) + bt
-  float lc_f, lc2m1; // compression transition lambda_c, lambda_c^2 - 1
-  float Kc, bc;    // compression extension: f = Kc*(lam - lc) + bc
+  float lc2m1;     // lambda_c^2 - 1
+  // extensions f = K (lam - L0) + b beyond lambda_t and lambda_c, as
+  // t = il (K - C / lam) with C = K L0 - b
+  float Kt, Ct;
+  float Kc, Cc;
   // alpha expm1(B/2 y) = y P(y) in the toe range: P's coefficients
   // alpha (B/2)^k / k!, k = 1..7 (the Taylor polynomial with both scalings
   // folded in)

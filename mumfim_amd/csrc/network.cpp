@@ -316,14 +316,29 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
         c = std::max(c, std::sqrt(p.reactions[r].E / p.reactions[r].rho));
     p.c_sound = c;
   }
-  // degrees in input order
-  std::vector<int> deg_in(N, 0);
+  // degrees in input order; in an RVE with fibers of both laws, the number of
+  // nonlinear incidences of each node
+  std::vector<char> fib_nl(M, 0);
+  int n_nl_fib = 0;
+  for (int e = 0; e < M; ++e) {
+    fib_nl[e] = p.reactions[p.rid[e]].type == MF_REACTION_NONLINEAR;
+    n_nl_fib += fib_nl[e];
+  }
+  const bool mixed = n_nl_fib > 0 && n_nl_fib < M;
+  std::vector<int> deg_in(N, 0), nl_in(N, 0);
   for (int e = 0; e < M; ++e) {
     deg_in[net.fib[2 * e]]++;
     deg_in[net.fib[2 * e + 1]]++;
+    if (mixed && fib_nl[e]) {
+      nl_in[net.fib[2 * e]]++;
+      nl_in[net.fib[2 * e + 1]]++;
+    }
   }
-  // kernel order: free nodes (sorted by degree desc, then Morton order),
-  // then boundary nodes in boundary-list order (BatchedReorderMesh.h:54-73)
+  // kernel order: free nodes (sorted by degree desc, then by nonlinear
+  // incidences desc, then Morton order), then boundary nodes in
+  // boundary-list order (BatchedReorderMesh.h:54-73).  The second key gives
+  // the lanes of a wave similar counts of nonlinear incidences, which the
+  // FP32 kernels list first.
   std::vector<int> free_nodes;
   free_nodes.reserve(N - Nb);
   for (int i = 0; i < N; ++i)
@@ -356,7 +371,9 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
     }
   }
   std::stable_sort(free_nodes.begin(), free_nodes.end(), [&](int a, int b) {
-    return deg_in[a] != deg_in[b] ? deg_in[a] > deg_in[b] : morton[a] < morton[b];
+    if (deg_in[a] != deg_in[b]) return deg_in[a] > deg_in[b];
+    if (nl_in[a] != nl_in[b]) return nl_in[a] > nl_in[b];
+    return morton[a] < morton[b];
   });
   p.input_of_rank.resize(N);
   p.rank_of_input.resize(N);
@@ -403,13 +420,20 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
     p.max_free_deg = std::max(p.max_free_deg, p.deg[r]);
     if (p.deg[r] > kLaneDegree) p.n_split++;
   }
-  // CSR in kernel order, entries in increasing fiber index
+  // CSR in kernel order, entries in increasing fiber index; with fibers of
+  // both laws, a node's nonlinear fibers first (each law in increasing fiber
+  // index)
   std::vector<int> row(N + 1, 0);
   for (int r = 0; r < N; ++r) row[r + 1] = row[r] + p.deg[r];
   std::vector<int> fillp(row.begin(), row.end() - 1);
   std::vector<int32_t> csr_w(row[N]), csr_e(row[N]);
   std::vector<double> csr_l0(row[N]);
-  for (int e = 0; e < M; ++e) {
+  std::vector<int> fib_order;
+  fib_order.reserve(M);
+  for (int pass = mixed ? 0 : 1; pass < 2; ++pass)
+    for (int e = 0; e < M; ++e)
+      if (!mixed || (fib_nl[e] != 0) == (pass == 0)) fib_order.push_back(e);
+  for (int e : fib_order) {
     int a = p.fib[2 * e], c = p.fib[2 * e + 1];
     int32_t tag = p.rid[e] << 24;
     csr_e[fillp[a]] = e;
@@ -479,6 +503,21 @@ int pack_rve(const Network &net, int reaction_policy, PackedRVE &p) {
         p.fstream.insert(p.fstream.end(),
                          {r, o, end_slot[2 * (size_t)e + end], end_slot[2 * (size_t)e + 1 - end]});
       }
+  if (mixed) {
+    // nonlinear fibers first (each law in the order above): a wave of the
+    // fiber-parallel kernels then holds fibers of one law, except the one
+    // wave at the border
+    const size_t nfree = p.fstream.size() / 4;
+    std::vector<int32_t> part;
+    part.reserve(p.fstream.size());
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t i = 0; i < nfree; ++i) {
+        const int32_t *q = &p.fstream[4 * i];
+        const bool nl = p.reactions[(uint32_t)p.ell_w[q[2]] >> 24].type == MF_REACTION_NONLINEAR;
+        if (nl == (pass == 0)) part.insert(part.end(), q, q + 4);
+      }
+    p.fstream.swap(part);
+  }
   for (int e = 0; e < M; ++e)
     if (!taken[e]) p.fstream.insert(p.fstream.end(), {p.fib[2 * e], p.fib[2 * e + 1], -1, -1});
   p.fparam.assign(4 * (size_t)M, 0.0f);

@@ -90,8 +90,11 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   const double inv_S = 1.0 / a.n_steps;
 
   float uh[NPT][3], ul[NPT][3], vr[NPT][3], imr[NPT];
-  // slice width and base are wave-uniform (scalar registers)
-  int Dr[NPT], sbase[NPT];
+  // slice width and base are wave-uniform (scalar registers); Dn: with one
+  // linear and one nonlinear reaction, the entries [0, Dn) hold the wave's
+  // nonlinear fibers (nodes list them first), later entries are linear on
+  // every lane
+  int Dr[NPT], sbase[NPT], Dn[NPT];
   // initial displacement: relax adds the affine increment (F - I) x to u,
   // ramp keeps u and sets the boundary increment aside.
   auto init_node = [&](int r, double (&u)[3]) {
@@ -142,6 +145,14 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     const int k = r / kWave;
     Dr[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
     sbase[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
+    int cnt = 0;
+    if (mix2 && r < Nf) {
+      const int dg = a.deg[nb + r];
+      while (cnt < dg && (int)((uint32_t)gw[sbase[s] + lane + cnt * kWave] >> 24) == nl_rid) ++cnt;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o, kWave));
+    Dn[s] = __builtin_amdgcn_readfirstlane(cnt);
   }
   for (int r = NPT * B + tid; r < N; r += B) {
     double u[3];
@@ -187,9 +198,15 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
         const float e2 = sl * (il * il);
         const unsigned rid = (uint32_t)wv >> 24;
-        const float t = lin_only ? lin_EA_half * il * e2
-                        : mix2   ? react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il)
-                                 : react32_tab(rtab, rid, e2, il);
+        float t;
+        if (lin_only)
+          t = lin_EA_half * il * e2;
+        else if (mix2 && j < Dn[s])
+          t = react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il);
+        else if (mix2)
+          t = ea_half_lin * il * e2;
+        else
+          t = react32_tab(rtab, rid, e2, il);
         fx -= dx * t;
         fy -= dy * t;
         fz -= dz * t;
@@ -560,6 +577,7 @@ void solve32r(SolveArgs a) {
   int offs[NPT][kLaneDeg];
   float cf[NPT][kLaneDeg], d0x[NPT][kLaneDeg], d0y[NPT][kLaneDeg], d0z[NPT][kLaneDeg];
   int Ds[NPT];
+  unsigned nlm[NPT];
 #pragma unroll
   for (int sl = 0; sl < NPT; ++sl) {
     const int r = sl == 0 ? (split ? tid >> 1 : tid - S) : sl * B - S + tid;
@@ -629,6 +647,17 @@ void solve32r(SolveArgs a) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) D = max(D, __shfl_xor(D, o, kWave));
     Ds[sl] = __builtin_amdgcn_readfirstlane(D);
+    // RM_MIX2: bit j set when a lane of the wave has a nonlinear fiber at
+    // entry j (nodes list their nonlinear fibers first and a wave's nodes
+    // have similar nonlinear counts, so the high entries are mostly linear
+    // for the whole wave and skip the toe law)
+    unsigned m = 0;
+    if (RM == RM_MIX2) {
+#pragma unroll
+      for (int j = 0; j < kLaneDeg; ++j)
+        m |= (__builtin_amdgcn_ballot_w64(cf[sl][j] < 0.0f) != 0 ? 1u : 0u) << j;
+    }
+    nlm[sl] = __builtin_amdgcn_readfirstlane(m);
   }
   // a later chunk continues from the state its predecessor saved
   float *cs = chunked ? a.cstate + (int64_t)unit * a.cstride : nullptr;
@@ -700,19 +729,14 @@ void solve32r(SolveArgs a) {
         if (RM == RM_LIN) {
           t = cf[sl][j] * sq;
         } else if (RM == RM_MIX2) {
-          // every lane evaluates both laws and selects (see react32_tab)
+          // a wave with a nonlinear fiber at entry j evaluates both laws
+          // on every lane and selects
           const float c = cf[sl][j];
           const float il = -c;
-          const float e2 = sq * (il * il);
-          const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-          const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
-          const float lam2 = 1.0f + e2;
-          const float ilam = __frsqrt_rn(lam2);
-          const float lam = lam2 * ilam;
-          const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
-          const float bb = hi ? nlq.bt : nlq.bc;
-          const float te = (K * (lam - L0) + bb) * il * ilam;
-          t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
+          if (nlm[sl] & (1u << j))
+            t = c < 0.0f ? toe_ext_t(nlq, sq * (il * il), il) : c * sq;
+          else
+            t = c * sq;
         } else {
           const float il = cf[sl][j];
           t = react32_tab(rtab, (uint32_t)offs[sl][j] >> 24, sq * (il * il), il);
@@ -1158,10 +1182,15 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
   // end forces of every owned fiber into the slots; squared current length
   // minimum over the owned fibers when asked
   // waves whose fibers of a slot all lie past M skip that slot
-  bool live[FPL];
+  // nlw: a fiber of the wave's slot is nonlinear (the stream lists an RVE's
+  // nonlinear fibers first, so most waves hold one law)
+  bool live[FPL], nlw[FPL];
 #pragma unroll
-  for (int s = 0; s < FPL; ++s)
+  for (int s = 0; s < FPL; ++s) {
     live[s] = __builtin_amdgcn_readfirstlane(s * B + (tid & ~(kWave - 1))) < M;
+    nlw[s] = __builtin_amdgcn_readfirstlane(
+                 __builtin_amdgcn_ballot_w64(cf[s] < 0.0f) != 0 ? 1 : 0) != 0;
+  }
   auto fiber_phase = [&](bool want_min) {
     const unsigned char *P = reinterpret_cast<const unsigned char *>(pos);
     unsigned char *Fs = reinterpret_cast<unsigned char *>(fsl);
@@ -1189,16 +1218,10 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       } else if (RM == RM_MIX2) {
         const float c = cf[s];
         const float il = -c;
-        const float e2 = sq * (il * il);
-        const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-        const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
-        const float lam2 = 1.0f + e2;
-        const float ilam = __frsqrt_rn(lam2);
-        const float lam = lam2 * ilam;
-        const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
-        const float bb = hi ? nlq.bt : nlq.bc;
-        const float te = (K * (lam - L0) + bb) * il * ilam;
-        t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
+        if (nlw[s])
+          t = c < 0.0f ? toe_ext_t(nlq, sq * (il * il), il) : c * sq;
+        else
+          t = c * sq;
       } else {
         const float il = cf[s];
         t = react32_tab(rtab, (uint32_t)pa[s] >> 24, sq * (il * il), il);
@@ -1583,7 +1606,9 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   // fibers: e = s * B + tid in stream order; endpoint byte offsets a | b << 16
   int pab[FPL];
   float d0x[FPL], d0y[FPL], d0z[FPL], cf[FPL];
-  bool live[FPL];
+  // nlw: a fiber of the wave's slot is nonlinear (the stream lists an RVE's
+  // nonlinear fibers first, so most waves hold one law)
+  bool live[FPL], nlw[FPL];
   const int32_t *fs = a.fstream + 4 * R.fib_off;
   const float4 *fp = reinterpret_cast<const float4 *>(a.fparam) + R.fib_off;
 #pragma unroll
@@ -1601,6 +1626,8 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       d0z[s] = c.z;
       cf[s] = c.w;
     }
+    nlw[s] = __builtin_amdgcn_readfirstlane(
+                 __builtin_amdgcn_ballot_w64(cf[s] < 0.0f) != 0 ? 1 : 0) != 0;
   }
   __syncthreads();
 
@@ -1628,16 +1655,10 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       } else {
         const float c = pc;
         const float il = -c;
-        const float e2 = sq * (il * il);
-        const bool hi = e2 > nlq.lt2m1, lo = e2 < nlq.lc2m1;
-        const float tt = il * toe_poly(nlq, fminf(fmaxf(e2, nlq.lc2m1), nlq.lt2m1));
-        const float lam2 = 1.0f + e2;
-        const float ilam = __frsqrt_rn(lam2);
-        const float lam = lam2 * ilam;
-        const float K = hi ? nlq.Kt : nlq.Kc, L0 = hi ? nlq.lt_f : nlq.lc_f;
-        const float bb = hi ? nlq.bt : nlq.bc;
-        const float te = (K * (lam - L0) + bb) * il * ilam;
-        t = c < 0.0f ? ((hi || lo) ? te : tt) : c * sq;
+        if (nlw[s])
+          t = c < 0.0f ? toe_ext_t(nlq, sq * (il * il), il) : c * sq;
+        else
+          t = c * sq;
       }
       const int e = s * B + tid;
       if (e < M) {

@@ -90,6 +90,40 @@ static void check(const PackedRVE &p, int t) {
   long free_ends = 0;
   for (int r = 0; r < Nf; ++r) free_ends += p.deg[r];
   if ((long)slots.size() != free_ends) fail("free ends without a slot", t);
+  // fibers of both laws: a node lists its nonlinear fibers first; free
+  // nodes run by degree, then by nonlinear count (both descending); the
+  // stream holds its nonlinear free fibers first
+  auto is_nl = [&](int at) {
+    return p.reactions[(uint32_t)p.ell_w[at] >> 24].type == MF_REACTION_NONLINEAR;
+  };
+  int n_nl = 0;
+  for (int e = 0; e < M; ++e) n_nl += p.reactions[p.rid[e]].type == MF_REACTION_NONLINEAR;
+  if (n_nl == 0 || n_nl == M) return;
+  int prev_deg = 1 << 30, prev_nl = 1 << 30;
+  for (int r = 0; r < Nf; ++r) {
+    const int k = r / kSlice;
+    int cnt = 0;
+    bool lin_seen = false;
+    for (int j = 0; j < p.deg[r]; ++j) {
+      const int at = p.slice_base[k] + j * kSlice + r % kSlice;
+      if (is_nl(at)) {
+        if (lin_seen) fail("nonlinear entry after a linear one", t);
+        ++cnt;
+      } else {
+        lin_seen = true;
+      }
+    }
+    if (p.deg[r] > prev_deg || (p.deg[r] == prev_deg && cnt > prev_nl)) fail("node order", t);
+    prev_deg = p.deg[r];
+    prev_nl = cnt;
+  }
+  bool lin_stream = false;
+  for (int e = 0; e < M; ++e) {
+    const int sa = p.fstream[4 * (size_t)e + 2];
+    if (sa < 0) break;
+    if (!is_nl(sa)) lin_stream = true;
+    else if (lin_stream) fail("nonlinear fiber after a linear one in the stream", t);
+  }
 }
 
 int main(int argc, char **argv) {
