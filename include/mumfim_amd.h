@@ -122,7 +122,10 @@ typedef struct mf_solver_params {
   int32_t n_steps;              /* ramp mode: steps per solve                 */
   int32_t max_steps;            /* relax mode: step cap (status STEP_CAP)     */
   int32_t dt_update_freq;       /* steps between dt / residual refresh (100)  */
-  int32_t block_size;           /* threads per solve; 0 = planner's choice    */
+  int32_t block_size;           /* threads per solve; 0 = planner's choice;
+                                 * binds each precision that has kernels of
+                                 * that size (FP32c 256-1024, FP64 256, 512,
+                                 * 1024)                                      */
   int32_t amplitude;            /* ramp mode: MF_AMP_* boundary amplitude     */
   double visc_damp_coeff;       /* eta, mass-proportional damping             */
   double crit_time_scale_factor;/* dt = factor * min(l) / c (0.8)             */

@@ -303,6 +303,19 @@ bool mix2_ok(const PackedRVE &p) {
   return n_nl <= 1;
 }
 
+// The batch's block_size binds the kernels of precision prec when that
+// precision has variants of that thread count (FP64 has none at 640 or 768
+// threads, so an FP32c-only size leaves the FP64 plan to the planner).
+bool block_ok(const mf_batch *b, int prec, const Variant &v) {
+  const int bs = b->p.block_size;
+  if (!bs || v.block == bs) return true;
+  int nv = 0;
+  const Variant *vs = variants(&nv);
+  for (int i = 0; i < nv; ++i)
+    if (vs[i].prec == prec && vs[i].block == bs && !(vs[i].flags & kTangentLinear)) return false;
+  return true;
+}
+
 // Whether variant v can run RVE p (threads, registers, LDS, reactions,
 // energy diagnostics) under the batch parameters.
 bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE &p,
@@ -311,11 +324,10 @@ bool variant_fits(const mf_batch *b, int prec, const Variant &v, const PackedRVE
   // solve: only tangent-linear solve sets (make_plan_tl) run them
   if (v.flags & kTangentLinear) return false;
   if (v.flags & kGlobalState)  // FP64, any size; also the FP32c fallback
-    return !(prec == MF_PREC_FP64 && b->p.energy_diagnostics) &&
-           (!b->p.block_size || v.block == b->p.block_size) &&
+    return !(prec == MF_PREC_FP64 && b->p.energy_diagnostics) && block_ok(b, prec, v) &&
            lds_bytes(v, p.N, p.Nb, p.M, p.ell_free_entries) <= kMaxLds;
   if (v.prec != prec) return false;
-  if (b->p.block_size && v.block != b->p.block_size) return false;
+  if (!block_ok(b, prec, v)) return false;
   const bool regs = (v.flags & kEllInRegs) != 0;
   if ((long)v.block * v.npt < p.Nf + (regs ? p.n_split : 0)) return false;
   if (regs && (p.max_free_deg > 2 * kLaneDegree || 2 * p.n_split > v.block ||

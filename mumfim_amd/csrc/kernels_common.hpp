@@ -351,7 +351,10 @@ struct Lin64 {
   X(1024, 4, 2, kLinearOnly) X(1024, 4, 2, kMix2)                            \
   X(1024, 6, 3, kLinearOnly) X(1024, 6, 3, kMix2)                            \
   X(1024, 7, 3, kLinearOnly) X(1024, 7, 3, kMix2)                            \
-  X(1024, 8, 3, kLinearOnly)
+  X(1024, 8, 3, kLinearOnly)                                                 \
+  X(768, 6, 3, kLinearOnly) X(768, 6, 3, kMix2)                              \
+  X(768, 8, 4, kLinearOnly) X(768, 8, 4, kMix2)                              \
+  X(768, 10, 4, kLinearOnly) X(768, 10, 4, kMix2)
 
 // Fiber-parallel FP32c variants: (threads, fibers per thread, free nodes per
 // thread, reaction mode flags kLinearOnly|kMix2, 0 = reaction table).

@@ -1506,6 +1506,7 @@ struct F3 {
 template <int B, int FPL, int NPL, int RM>
 __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   constexpr bool LIN = RM == RM_LIN;
+  constexpr bool RR = B <= 768;
   constexpr int NW = B / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
@@ -1582,6 +1583,12 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   // free nodes: r = s * B + tid
   float uh[NPL][3], ul[NPL][3], vr[NPL][3], imr[NPL];
   int Dn[NPL], sbn[NPL];
+  // RR (up to 768 threads, whose register budget holds them): the first
+  // kLaneDeg force-record references of each node slot in registers, two
+  // 16-bit fields per register (record index, bit 15 set where the node is
+  // the fiber's end a and takes -force)
+  constexpr int kRw = (kLaneDeg + 1) / 2;
+  uint32_t rpk[NPL][RR ? kRw : 1];
 #pragma unroll
   for (int s = 0; s < NPL; ++s) {
     const int r = s * B + tid;
@@ -1601,6 +1608,16 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     const int k = r / kWave;
     Dn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
     sbn[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
+    if constexpr (RR) {
+#pragma unroll
+      for (int w = 0; w < kRw; ++w) rpk[s][w] = 0;
+#pragma unroll
+      for (int j = 0; j < kLaneDeg; ++j) {
+        const int32_t fr = (r < Nf && j < Dn[s]) ? gfr[sbn[s] + lane + j * kWave] : M;
+        const uint32_t field = ((uint32_t)fr & 0x7FFFu) | (fr < 0 ? 0x8000u : 0u);
+        rpk[s][j / 2] |= field << (16 * (j & 1));
+      }
+    }
   }
 
   // fibers: e = s * B + tid in stream order; endpoint byte offsets a | b << 16
@@ -1669,18 +1686,63 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     return lmin;
   };
   // internal force of a free node: -force of fibers where it is end a,
-  // +force where it is end b; references prefetched one entry ahead
+  // +force where it is end b, summed in list order.  RR: the first kLaneDeg
+  // references come from registers and the wave-uniform slice width selects
+  // a straight-line body, so the record loads issue together; entries past
+  // kLaneDeg (high-degree slices only) stream their references from global
+  // memory.  Otherwise the references stream from global memory one entry
+  // ahead.
   auto node_force = [&](int s, float (&f)[3]) {
-    const F3 *Fb = ffr;
-    const int32_t *E = gfr + sbn[s] + lane;
     const int D = Dn[s];
+    const int32_t *E = gfr + sbn[s] + lane;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (D > 0) {
+    if constexpr (RR) {
+      // opaque to the compiler: decoding the references ahead of the step
+      // loop would hold every decoded address and sign in registers
+#pragma unroll
+      for (int w = 0; w < kRw; ++w) asm volatile("" : "+v"(rpk[s][w]));
+      auto body = [&](auto KC) {
+        constexpr int K = decltype(KC)::value;
+        F3 v[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const uint32_t field = (rpk[s][j / 2] >> (16 * (j & 1))) & 0xFFFFu;
+          v[j] = ffr[field & 0x7FFFu];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const uint32_t field = (rpk[s][j / 2] >> (16 * (j & 1))) & 0xFFFFu;
+          const float sg = __uint_as_float(0x3F800000u | ((field & 0x8000u) << 16));
+          fx = fmaf(sg, v[j].x, fx);
+          fy = fmaf(sg, v[j].y, fy);
+          fz = fmaf(sg, v[j].z, fz);
+        }
+      };
+      switch (D < kLaneDeg ? D : kLaneDeg) {
+#define MF_DEG_CASE(k) \
+  case k:              \
+    body(std::integral_constant<int, k>{}); \
+    break;
+        MF_DEG_CASE(1) MF_DEG_CASE(2) MF_DEG_CASE(3)
+        MF_DEG_CASE(4) MF_DEG_CASE(5) MF_DEG_CASE(6)
+#undef MF_DEG_CASE
+        default:
+          break;
+      }
+      for (int j = kLaneDeg; j < D; ++j) {
+        const int32_t fr = E[j * kWave];
+        const F3 v = ffr[fr & 0x7FFFFFFF];
+        const float sg = fr < 0 ? -1.0f : 1.0f;
+        fx = fmaf(sg, v.x, fx);
+        fy = fmaf(sg, v.y, fy);
+        fz = fmaf(sg, v.z, fz);
+      }
+    } else if (D > 0) {
       int32_t nx = E[0];
       for (int j = 0; j < D; ++j) {
         const int32_t fr = nx;
         if (j + 1 < D) nx = E[(j + 1) * kWave];
-        const F3 v = Fb[fr & 0x7FFFFFFF];
+        const F3 v = ffr[fr & 0x7FFFFFFF];
         const float sg = __int_as_float((fr & (int32_t)0x80000000) | 0x3F800000);
         fx = fmaf(sg, v.x, fx);
         fy = fmaf(sg, v.y, fy);

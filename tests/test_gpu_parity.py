@@ -668,6 +668,36 @@ def test_fiber_kernels_match_oracle():
     assert np.abs(u_f - u_n).max() <= 1e-4 * np.abs(u_n).max()
 
 
+def test_fiber_streamed_variants_agree_bitwise():
+    """The fiber-streamed FP32c kernel at 768 threads (each node's first
+    force-record references held in registers) and at 1024 threads
+    (references streamed from global memory) evaluates every fiber the same
+    way and sums each node's records in the same list order: bitwise the
+    same stress and state, within the FP32c bar of the oracle.  Toe-law and
+    linear RVEs of 1400-2400 nodes (3-4 nodes and 6-8 fibers per thread)."""
+    say("case: fiber-streamed kernel, 768 vs 1024 threads")
+    nets = [mf.generate_network(seed=95 + i, n_nodes=n, nonlinear_fraction=nl)
+            for i, (n, nl) in enumerate([(1400, 0.5), (2400, 0.5), (2400, 0.0)])]
+    F = strain_F(12, len(nets), 0.08)
+    common = dict(n_steps=1000, reaction_policy=mf.MF_REACT_PER_FIBER)
+    out = {}
+    for block in (768, 1024):
+        g = mf.BatchedFiberRVEAnalysisExplicit(
+            nets, ramp_params(precision=mf.MF_PREC_FP32C, fp32_kernel=mf.MF_FP32_KERNEL_FIBER,
+                              block_size=block, **common))
+        _, plan = g.plan()
+        say(f"  {block} threads: plan {plan[:, :3].tolist()}")
+        assert all(int(f) & 128 for f in plan[:, 2]), plan  # fiber-streamed
+        s, st = g.run(F, return_status=True)
+        assert (st == mf.MF_STATUS_OK).all()
+        out[block] = (s, g.get_state()[1])
+    assert np.array_equal(out[768][0], out[1024][0])
+    assert np.array_equal(out[768][1], out[1024][1])
+    e = rel_err(out[768][0], Oracle(nets, ramp_params(**common)).run(F))
+    say(f"  vs oracle {e:.2e}")
+    assert e <= FP32C_TOL
+
+
 def test_large_rve_global_state_kernel():
     """An RVE with more free nodes (~5400) than the register and LDS kernels
     have lanes takes the FP64 kernel with the state in global memory: it
