@@ -384,8 +384,10 @@ int make_plan(mf_batch *b, int prec) {
       const int mode_rank = (v.flags & kLinearOnly) ? 0 : (v.flags & kMix2) ? 1 : 2;
       // Measured order.  Register-resident lists with one node per thread.
       // Then, for RVEs with nonlinear fibers: fiber-parallel with up to 3
-      // fibers per thread, fiber-streamed, fiber-parallel with more fibers,
-      // register lists with several nodes per thread; for all-linear RVEs
+      // fibers per thread, fiber-streamed (the shapes of up to 768 threads,
+      // which hold the node lists in registers, first: the autotuner picks
+      // them for 1800-3100-node toe-law RVEs), fiber-parallel with more
+      // fibers, register lists with several nodes per thread; for all-linear RVEs
       // the register lists with several nodes per thread come first.  Last
       // the LDS/global-list kernels, fewest node slots per thread first (more
       // waves hide the FP64 division and square-root chains: 1000-2000 node
@@ -409,7 +411,8 @@ int make_plan(mf_batch *b, int prec) {
                  : (v.flags & kFiberOnce)
                      ? (lin ? 1500000000L : 500000000L) + (long)v.npt * 10000000 + (long)v.block * 1000
                  : fpar ? (lin ? 50000000L : 15000000L) + fpl * 100000 + (long)v.block * 10 + mode_rank
-                 : fstr ? (lin ? 55000000L : 15350000L) + (long)v.npt * 10000 + fpl * 100 + mode_rank
+                 : fstr ? (lin ? 55000000L : 15350000L) + (v.block > 768 ? 50000L : 0L) +
+                              (long)v.npt * 10000 + fpl * 100 + mode_rank
                  : reg256x2 ? 10000000L + 5000
                  : regbig_nl ? 15340000L + mode_rank
                  : regs ? (long)v.npt * 10000000L + (long)v.block * 10 + mode_rank

@@ -165,8 +165,11 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   int cur = 0;
   // internal force of a slot's node.  ELL padding names the node itself
   // with 1/l0 = 0 (zero force), so every entry of the slice width is
-  // evaluated unmasked; entry j+1's list word and 1/l0 load while entry j is
-  // evaluated (rolling prefetch: the lists stream from HBM for large RVEs).
+  // evaluated unmasked; the list words and 1/l0 of entries j+1 and j+2 load
+  // while entry j is evaluated (rolling prefetch: the lists stream from HBM
+  // for large RVEs).  Both ends enter as their FP32-rounded displacements,
+  // like the register and fiber kernels (the low parts only keep the
+  // accumulation exact).
   auto node_force = [&](int s, int buf, float (&f)[3]) {
     const float4 *P = pos + (kDbuf ? buf * Np : 0);
     const int4 Xv = Xs[s * B + tid];  // own reference coordinate
@@ -174,14 +177,20 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int D = Dr[s];
     if (D > 0) {
-      int32_t wv_n = Ew[base];
-      float il_n = Eil[base];
+      int32_t wv_n = Ew[base], wv_nn = 0;
+      float il_n = Eil[base], il_nn = 0.0f;
+      if (D > 1) {
+        wv_nn = Ew[base + kWave];
+        il_nn = Eil[base + kWave];
+      }
       for (int j = 0; j < D; ++j) {
         const int32_t wv = wv_n;
         const float il = il_n;
-        if (j + 1 < D) {
-          wv_n = Ew[base + (j + 1) * kWave];
-          il_n = Eil[base + (j + 1) * kWave];
+        wv_n = wv_nn;
+        il_n = il_nn;
+        if (j + 2 < D) {
+          wv_nn = Ew[base + (j + 2) * kWave];
+          il_nn = Eil[base + (j + 2) * kWave];
         }
         const int w = wv & 0xFFFFFF;
         const float4 q = ld16(P + w);
@@ -190,9 +199,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const float ex = (float)(xq.x - Xv.x) * xq_inv;
         const float ey = (float)(xq.y - Xv.y) * xq_inv;
         const float ez = (float)(xq.z - Xv.z) * xq_inv;
-        const float gx = (q.x - uh[s][0]) - ul[s][0];
-        const float gy = (q.y - uh[s][1]) - ul[s][1];
-        const float gz = (q.z - uh[s][2]) - ul[s][2];
+        const float gx = q.x - uh[s][0];
+        const float gy = q.y - uh[s][1];
+        const float gz = q.z - uh[s][2];
         const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
         // l^2 - l0^2 = (d0 + d) . delta
         const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;

@@ -674,10 +674,11 @@ def test_fiber_streamed_variants_agree_bitwise():
     (references streamed from global memory) evaluates every fiber the same
     way and sums each node's records in the same list order: bitwise the
     same stress and state, within the FP32c bar of the oracle.  Toe-law and
-    linear RVEs of 1400-2400 nodes (3-4 nodes and 6-8 fibers per thread)."""
+    linear RVEs of 1800-2400 nodes (4400-5900 fibers: more than the
+    fiber-parallel shapes hold; 3-4 nodes and 6-8 fibers per thread)."""
     say("case: fiber-streamed kernel, 768 vs 1024 threads")
     nets = [mf.generate_network(seed=95 + i, n_nodes=n, nonlinear_fraction=nl)
-            for i, (n, nl) in enumerate([(1400, 0.5), (2400, 0.5), (2400, 0.0)])]
+            for i, (n, nl) in enumerate([(1800, 0.5), (2400, 0.5), (2400, 0.0)])]
     F = strain_F(12, len(nets), 0.08)
     common = dict(n_steps=1000, reaction_policy=mf.MF_REACT_PER_FIBER)
     out = {}
