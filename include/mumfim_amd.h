@@ -124,7 +124,7 @@ typedef struct mf_solver_params {
   int32_t dt_update_freq;       /* steps between dt / residual refresh (100)  */
   int32_t block_size;           /* threads per solve; 0 = planner's choice;
                                  * binds each precision that has kernels of
-                                 * that size (FP32c 256-1024, FP64 256, 512,
+                                 * that size (FP32c 64-1024, FP64 256, 512,
                                  * 1024)                                      */
   int32_t amplitude;            /* ramp mode: MF_AMP_* boundary amplitude     */
   double visc_damp_coeff;       /* eta, mass-proportional damping             */
@@ -152,11 +152,15 @@ typedef struct mf_solver_params {
 /* FP32c kernel families.  Node-centric kernels gather each free node's
  * fibers (every fiber is evaluated from both ends); fiber-parallel kernels
  * evaluate each fiber once and let nodes sum the published end forces (fewer
- * arithmetic operations, more LDS traffic; RVEs up to ~1800 nodes).  AUTO
- * lets the planner and autotuner choose per RVE. */
+ * arithmetic operations, more LDS traffic; RVEs up to ~1800 nodes).  WAVE
+ * restricts the node-centric family to its one-RVE-per-wave shapes (each
+ * wave solves its own RVE of up to ~200 nodes with no workgroup barrier;
+ * block_size 64 W packs W RVEs per workgroup).  AUTO lets the planner and
+ * autotuner choose per RVE. */
 #define MF_FP32_KERNEL_AUTO 0
 #define MF_FP32_KERNEL_NODE 1
 #define MF_FP32_KERNEL_FIBER 2
+#define MF_FP32_KERNEL_WAVE 3
 
 /* Ramp-mode boundary amplitude alpha(xi), xi = k / n_steps. */
 #define MF_AMP_LINEAR 0         /* alpha = xi                                */

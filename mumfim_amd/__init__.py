@@ -26,7 +26,7 @@ MF_REACTION_LINEAR, MF_REACTION_NONLINEAR = 0, 1
 MF_STATUS_OK, MF_STATUS_NAN, MF_STATUS_STEP_CAP, MF_STATUS_FALLBACK = 0, 1, 2, 3
 MF_STATUS_ENERGY = 4
 MF_AMP_LINEAR, MF_AMP_SMOOTHSTEP = 0, 1
-MF_FP32_KERNEL_AUTO, MF_FP32_KERNEL_NODE, MF_FP32_KERNEL_FIBER = 0, 1, 2
+MF_FP32_KERNEL_AUTO, MF_FP32_KERNEL_NODE, MF_FP32_KERNEL_FIBER, MF_FP32_KERNEL_WAVE = 0, 1, 2, 3
 
 
 class mf_reaction(C.Structure):

@@ -171,6 +171,12 @@ static const Variant kVariants[] = {
     {MF_PREC_FP32C, B, 1, kEllInRegs | kDoubleBuf | kMix2},
     MF_REG2(256) MF_REG2(512) MF_REG2(640) MF_REG2(768) MF_REG2(1024)
 #undef MF_REG2
+    // one RVE per wave, W RVEs per workgroup: (W, slots per thread)
+#define MF_WAVE_ENTRY(W, N)                                                        \
+    {MF_PREC_FP32C, 64 * (W), N, kEllInRegs | kDoubleBuf | kLinearOnly | kWaveRve}, \
+    {MF_PREC_FP32C, 64 * (W), N, kEllInRegs | kDoubleBuf | kMix2 | kWaveRve},
+    MF_WAVE_LIST(MF_WAVE_ENTRY)
+#undef MF_WAVE_ENTRY
     {MF_PREC_FP32C, 256, 1, kEllInRegs | kDoubleBuf},
     {MF_PREC_FP32C, 512, 1, kEllInRegs | kDoubleBuf},
     // several free nodes per thread
@@ -210,7 +216,7 @@ const Variant *variants(int *count) {
 
 size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
   const size_t Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
-  const size_t NW = v.block / kern::kWave;
+  const size_t NW = (v.flags & kWaveRve) ? 1 : v.block / kern::kWave;  // waves per RVE
   size_t b = 0;
   if (v.flags & kFiberOnce) {
     // double-buffered positions, t per fiber (M bounds the fibers with a
@@ -246,7 +252,9 @@ size_t lds_bytes(const Variant &v, int N, int Nb, int M, int ell_free_entries) {
     // low parts at the end
     b += 16 * (Np + std::max<size_t>((size_t)ell_free_entries + kern::kWave, Np));
   } else if (v.flags & kEllInRegs) {
-    b += 16 * (kRegBufStride + Np);  // displacement buffers only, fixed stride
+    // displacement buffers only, fixed stride (per RVE: with one RVE per
+    // wave the launch multiplies this by the RVEs per workgroup)
+    b += 16 * (((v.flags & kWaveRve) ? kWaveBufStride : kRegBufStride) + Np);
   } else {
     // displacement buffers + reference coordinates
     b += 16 * Np * ((v.flags & kDoubleBuf) ? 3 : 2);

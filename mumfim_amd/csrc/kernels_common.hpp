@@ -346,6 +346,11 @@ struct Lin64 {
 // ones spill a few registers in the step loop (7 toe-law or 8 linear fibers
 // per thread) and still beat the LDS-list kernel; 8 toe-law fibers per
 // thread did not.
+// One-RVE-per-wave FP32c register variants: (RVEs per workgroup, free-node
+// slots per thread), each in the linear and one-linear-one-nonlinear modes
+// (four slots per thread spill registers).
+#define MF_WAVE_LIST(X) X(1, 3) X(4, 3)
+
 #define MF_FIBER_STREAM_LIST(X)                                              \
   X(512, 4, 2, kLinearOnly) X(512, 4, 2, kMix2)                              \
   X(1024, 4, 2, kLinearOnly) X(1024, 4, 2, kMix2)                            \

@@ -44,7 +44,9 @@ struct SolveArgs {
   // NULL: eta for every RVE
   const double *eta_rve;
   int32_t amplitude;        // MF_AMP_* (ramp mode)
-  int32_t pad_;
+  // one-RVE-per-wave variants: bytes of LDS per wave (each wave's region
+  // starts at wave * wave_lds)
+  int32_t wave_lds;
   double energy_eps;        // > 0: energy balance check (energy variants)
   double *energy;           // 4 per solve (energy variants), may be NULL
   // global-state variants: per-solve scratch at gstate + block * gstride
@@ -86,6 +88,7 @@ enum : int32_t {
   kMix2 = 32,       // at most one linear and one nonlinear reaction (FP32c registers)
   kFiberPar = 64,   // FP32c fiber-parallel: lanes own fibers, nodes sum end forces
   kFiberStream = 128,  // FP32c fiber-streamed: per-fiber forces, nodes gather by reference
+  kWaveRve = 4096,     // FP32c registers, one RVE per wave, block / 64 RVEs per workgroup
   kGlobalState = 8192, // FP64 state in global memory: any RVE size
   kFiberOnce = 16384,  // FP64, t = f/l once per fiber, nodes gather by reference
   kTangentLinear = 32768,  // FP32c solve with its derivative along solve_G
@@ -94,6 +97,9 @@ enum : int32_t {
 // kernel: buffer 1 starts 32 KiB into LDS, a compile-time offset the LDS
 // loads take as an immediate; RVEs of up to this many nodes.
 constexpr int kRegBufStride = 2048;
+// The same stride of the one-RVE-per-wave variants, whose LDS region per
+// wave holds RVEs of up to this many nodes.
+constexpr int kWaveBufStride = 384;
 // Fibers per thread of a fiber-parallel variant (flags bits 8-11; the
 // other flags stay clear of them).
 constexpr int kFplShift = 8;

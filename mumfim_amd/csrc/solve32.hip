@@ -475,27 +475,62 @@ constexpr int kLaneDeg = kLaneDegree;
 // threads: two, up to 256), except 256 threads with two nodes per thread,
 // which runs three solves per CU (three waves, up to 168 VGPRs); the other
 // one-node shapes are left to the compiler.
-template <int B, int NPT, int RM>
+// One RVE per wave (W > 0, W RVEs per workgroup of 64 W threads): two
+// waves per SIMD (up to 256 VGPRs for three or four slots).
+template <int B, int NPT, int RM, int W>
 constexpr int reg_waves_min() {
-  return RM == RM_TABLE ? 2 : NPT == 1 ? (B <= 512 ? 4 : 1) : (B == 768 || B == 256 ? 3 : 2);
+  return W > 0 ? 2
+         : RM == RM_TABLE ? 2 : NPT == 1 ? (B <= 512 ? 4 : 1) : (B == 768 || B == 256 ? 3 : 2);
 }
-template <int B, int NPT, int RM>
+template <int B, int NPT, int RM, int W>
 constexpr int reg_waves_max() {
-  return RM == RM_TABLE || NPT > 1 ? reg_waves_min<B, NPT, RM>() : 8;
+  return W > 0 || RM == RM_TABLE || NPT > 1 ? reg_waves_min<B, NPT, RM, W>() : 8;
 }
 
-template <int B, int NPT, int RM>
+// Orders the LDS accesses of one wave: its LDS operations execute in issue
+// order, so only the compiler's code motion has to be fenced.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// W = 0: one RVE per workgroup of B threads.  W > 0 (B = 64 W): each wave
+// solves its own small RVE in its own LDS region (a.wave_lds bytes, buffer
+// stride kWaveBufStride), synchronising only within the wave, so a
+// workgroup carries W RVEs and no workgroup barrier.
+template <int B, int NPT, int RM, int W>
 __global__ __launch_bounds__(B)
-__attribute__((amdgpu_waves_per_eu(reg_waves_min<B, NPT, RM>(), reg_waves_max<B, NPT, RM>())))
+__attribute__((amdgpu_waves_per_eu(reg_waves_min<B, NPT, RM, W>(), reg_waves_max<B, NPT, RM, W>())))
 void solve32r(SolveArgs a) {
   constexpr bool LIN = RM == RM_LIN;
-  constexpr int NW = B / kWave;
+  constexpr bool WM = W > 0;
+  constexpr int TB = WM ? kWave : B;  // threads per RVE
+  constexpr int BS = WM ? kWaveBufStride : kRegBufStride;
+  constexpr int NW = TB / kWave;
+  static_assert(!WM || B == W * kWave, "one RVE per wave");
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tid = threadIdx.x;
+  const int wave = WM ? (int)(threadIdx.x / kWave) : 0;
+  const int tid = WM ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
+  auto sync = [&]() __attribute__((always_inline)) {
+    if constexpr (WM) wave_sync();
+    else __syncthreads();
+  };
+  // min of a and sum of b over the RVE's threads (every thread receives both)
+  double *red = nullptr;
+  auto red_min_sum = [&](double &x, double &y) __attribute__((always_inline)) {
+    if constexpr (WM) {
+      x = wave_min(x);
+      y = wave_sum(y);
+    } else {
+      block_min_sum<B>(x, y, red);
+    }
+  };
   // step chunk of this workgroup (ramp mode): chunk `chunk` of the group's
-  // solve `unit`
-  int unit = blockIdx.x, chunk = 0;
-  const bool chunked = a.n_chunks > 1;
+  // solve `unit` (no chunks with one RVE per wave)
+  int unit = WM ? (int)blockIdx.x * W + wave : (int)blockIdx.x, chunk = 0;
+  if (WM && unit >= a.group_n) return;  // the last workgroup's spare waves
+  const bool chunked = !WM && a.n_chunks > 1;
   if (chunked) {
     chunk = unit / a.group_n;
     unit -= chunk * a.group_n;
@@ -515,16 +550,20 @@ void solve32r(SolveArgs a) {
   const int Np = (N + 1) & ~1, Nbp = (Nb + 1) & ~1;
   const int64_t nb = R.node_off;
 
+  // this RVE's LDS region (dynamic LDS starts at 0, so `region` is also
+  // its byte address)
+  const int region = WM ? wave * a.wave_lds : 0;
+  unsigned char *const lds = smem + region;
   size_t off = 0;
   // displacement hi, two buffers at a fixed stride: the buffer of a step is a
   // compile-time constant, so it folds into the LDS loads' immediate offset
-  float4 *pos = reinterpret_cast<float4 *>(smem);  // [2][kRegBufStride]
-  off += sizeof(float4) * (kRegBufStride + Np);
-  double *red = reinterpret_cast<double *>(smem + off);
+  float4 *pos = reinterpret_cast<float4 *>(lds);  // [2][BS]
+  off += sizeof(float4) * (BS + Np);
+  red = reinterpret_cast<double *>(lds + off);
   off += sizeof(double) * NW * kRedStride;
-  double *bst = reinterpret_cast<double *>(smem + off);  // [Nbp][6] u_b0, g
+  double *bst = reinterpret_cast<double *>(lds + off);  // [Nbp][6] u_b0, g
   off += sizeof(double) * 6 * Nbp;
-  float4 *rtab = reinterpret_cast<float4 *>(smem + off);  // packed reaction table
+  float4 *rtab = reinterpret_cast<float4 *>(lds + off);  // packed reaction table
 
   const int32_t *gw = a.ell_w + R.ell_off;
   const float *gil = a.ell_il + R.ell_off;
@@ -563,7 +602,7 @@ void solve32r(SolveArgs a) {
     for (int k = 0; k < 3; ++k) u[k] = ramp ? up[k] : inc[k] + up[k];
     const float4 p = make_float4((float)u[0], (float)u[1], (float)u[2], 0.0f);
     pos[r] = p;
-    pos[kRegBufStride + r] = p;
+    pos[BS + r] = p;
     if (r >= Nf) {
       const int b = r - Nf;
 #pragma unroll
@@ -572,7 +611,7 @@ void solve32r(SolveArgs a) {
       bnd_set(bst + 6 * b, u, inc);
     }
   };
-  for (int rb = tid; rb < N; rb += B) {
+  for (int rb = tid; rb < N; rb += TB) {
     double u[3];
     init_node(rb, u);
   }
@@ -589,7 +628,7 @@ void solve32r(SolveArgs a) {
   unsigned nlm[NPT];
 #pragma unroll
   for (int sl = 0; sl < NPT; ++sl) {
-    const int r = sl == 0 ? (split ? tid >> 1 : tid - S) : sl * B - S + tid;
+    const int r = sl == 0 ? (split ? tid >> 1 : tid - S) : sl * TB - S + tid;
     rr[sl] = r;
     own[sl] = r < Nf;
     writer[sl] = own[sl] && (sl > 0 || !split || !(tid & 1));
@@ -624,7 +663,7 @@ void solve32r(SolveArgs a) {
         cnt = dg;
       }
     }
-    const int self = (own[sl] ? r : 0) * (int)sizeof(float4);
+    const int self = region + (own[sl] ? r : 0) * (int)sizeof(float4);
 #pragma unroll
     for (int j = 0; j < kLaneDeg; ++j) {
       offs[sl][j] = self;
@@ -634,7 +673,7 @@ void solve32r(SolveArgs a) {
         const int idx = base + (j0 + j) * kWave;
         const int32_t wv = gw[idx];
         const int w = wv & 0xFFFFFF;
-        offs[sl][j] = w * (int)sizeof(float4) | (RM == RM_TABLE ? (wv & 0xFF000000) : 0);
+        offs[sl][j] = (region + w * (int)sizeof(float4)) | (RM == RM_TABLE ? (wv & 0xFF000000) : 0);
         const double l0 = gl0[idx];
         if (RM == RM_LIN) {
           // EA/(2 l0^3 m): the gathered force is the acceleration
@@ -683,7 +722,7 @@ void solve32r(SolveArgs a) {
       }
       if (!ok) cs[NPT * 9 * B + 1] = __builtin_nanf("");  // reported as NaN at the end
     }
-    __syncthreads();
+    sync();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every lane sees the saved state
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl)
@@ -696,7 +735,7 @@ void solve32r(SolveArgs a) {
     h_saved = cs[NPT * 9 * B];
     res_saved = cs[NPT * 9 * B + 1];
   }
-  __syncthreads();
+  sync();
 
   const float inv_c = (float)(1.0 / R.c_sound);
   const float eta = (float)(a.eta_rve ? a.eta_rve[rv] : a.eta);
@@ -711,7 +750,7 @@ void solve32r(SolveArgs a) {
     // static LDS, checked on entry), so neighbour offset + buffer constant is
     // the whole address and the constant becomes the load's immediate offset
     const lds_uchar *P = reinterpret_cast<const lds_uchar *>(
-        static_cast<uintptr_t>(buf * kRegBufStride * (int)sizeof(float4)));
+        static_cast<uintptr_t>(buf * BS * (int)sizeof(float4)));
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto body = [&](auto KC) {
       constexpr int K = decltype(KC)::value;
@@ -783,10 +822,10 @@ void solve32r(SolveArgs a) {
   };
   // min fiber length^2 over all fibers (every dt refresh)
   auto min_length2 = [&](int buf) {
-    const float4 *P = pos + buf * kRegBufStride;
+    const float4 *P = pos + buf * BS;
     float lmin = 3.0e38f;
     const int32_t *fb = a.fib + 2 * R.fib_off;
-    for (int e = tid; e < M; e += B) {
+    for (int e = tid; e < M; e += TB) {
       const int n1 = fb[2 * e], n2 = fb[2 * e + 1];
       const float4 p1 = P[n1], p2 = P[n2];
       const float x1 = (float)(X[3 * n2] - X[3 * n1]) + (p2.x - p1.x);
@@ -815,7 +854,7 @@ void solve32r(SolveArgs a) {
     for (int sl = 0; sl < NPT; ++sl)
       if (own[sl]) node_force(sl, std::integral_constant<int, 0>{}, fint[sl], true);
     double lmin2 = min_length2(cur);
-    block_min_sum<B>(lmin2, resid_local, red);
+    red_min_sum(lmin2, resid_local);
     const float h = crit_half * (sqrtf((float)lmin2) * inv_c);
     const float hk = h_prev + h;
 #pragma unroll
@@ -838,7 +877,7 @@ void solve32r(SolveArgs a) {
   auto one_step = [&](auto NXTC) __attribute__((always_inline)) -> bool {
     constexpr int nxt = decltype(NXTC)::value;
     const float h2 = 2.0f * h_prev;
-    float4 *Pn = pos + nxt * kRegBufStride;
+    float4 *Pn = pos + nxt * BS;
 #pragma unroll
     for (int sl = 0; sl < NPT; ++sl) {
       if (own[sl]) {
@@ -859,13 +898,13 @@ void solve32r(SolveArgs a) {
     }
     if (ramp && tid < Nb) {  // waves without boundary nodes skip the amplitude
       const double alpha = ramp_alpha_step(step + 1, inv_S, a.amplitude);
-      for (int b = tid; b < Nb; b += B) {
+      for (int b = tid; b < Nb; b += TB) {
         float ub[3];
         bnd_step(bst + 6 * b, alpha, ub);
         Pn[Nf + b] = make_float4(ub[0], ub[1], ub[2], 0.0f);
       }
     }
-    __syncthreads();
+    sync();
     cur = nxt;
     const bool refresh = (step % freq) == 0;
     double rsum = 0.0;
@@ -884,7 +923,7 @@ void solve32r(SolveArgs a) {
     float h = h_prev;
     if (refresh) {
       double lm = min_length2(nxt);
-      block_min_sum<B>(lm, rsum, red);
+      red_min_sum(lm, rsum);
       h = crit_half * (sqrtf((float)lm) * inv_c);
       residual = sqrt(rsum);
     }
@@ -928,7 +967,7 @@ void solve32r(SolveArgs a) {
       cs[NPT * 9 * B + 1] = (float)residual;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
+    sync();
     if (tid == 0)
       __hip_atomic_store(a.cflag + sid, chunk + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
@@ -938,14 +977,14 @@ void solve32r(SolveArgs a) {
 
   // finalize: low parts into the spare buffer, boundary gathers in FP32 with
   // exactly rounded reference vectors, stress in FP64
-  float4 *Ph = pos + cur * kRegBufStride;
-  float4 *Pl = pos + (cur ^ 1) * kRegBufStride;
-  __syncthreads();
+  float4 *Ph = pos + cur * BS;
+  float4 *Pl = pos + (cur ^ 1) * BS;
+  sync();
 #pragma unroll
   for (int sl = 0; sl < NPT; ++sl)
     if (writer[sl]) Pl[rr[sl]] = make_float4(ul[sl][0], ul[sl][1], ul[sl][2], 0.0f);
   const double alpha_end = ramp ? ramp_alpha(step, a.n_steps, a.amplitude) : 0.0;
-  for (int b = tid; b < Nb; b += B) {
+  for (int b = tid; b < Nb; b += TB) {
     double ub[3];
     bnd_disp(bst + 6 * b, alpha_end, ub);
     const float4 hb = make_float4((float)ub[0], (float)ub[1], (float)ub[2], 0.0f);
@@ -953,10 +992,10 @@ void solve32r(SolveArgs a) {
     Pl[Nf + b] = make_float4((float)(ub[0] - (double)hb.x), (float)(ub[1] - (double)hb.y),
                              (float)(ub[2] - (double)hb.z), 0.0f);
   }
-  __syncthreads();
+  sync();
   double st[6] = {0, 0, 0, 0, 0, 0};
   const int flags = a.solve_flags[sid];
-  for (int b = tid; b < Nb; b += B) {
+  for (int b = tid; b < Nb; b += TB) {
     const int rb = Nf + b;
     const int64_t gr = nb + rb;
     double ub[3], xb[3];
@@ -1014,7 +1053,12 @@ void solve32r(SolveArgs a) {
         if (flags & kWriteU) a.u_out[3 * g + d] = u;
       }
     }
-  block_sum<B, 6>(st, red);
+  if constexpr (WM) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st[i] = wave_sum(st[i]);
+  } else {
+    block_sum<B, 6>(st, red);
+  }
   if (tid == 0) {
     const double sv = R.scale / a.V[rv];
     bool bad = false;
@@ -1971,11 +2015,11 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
 }  // namespace kern
 
 int chunk_occupancy(const Variant &v, size_t lds) {
-  if (v.prec != MF_PREC_FP32C || !(v.flags & kEllInRegs)) return 0;
+  if (v.prec != MF_PREC_FP32C || !(v.flags & kEllInRegs) || (v.flags & kWaveRve)) return 0;
   int n = 0;
 #define MF_REG_OCC(B, N, RMODE, FLAG)                                          \
   if (v.npt == N && v.block == B && (v.flags & (kLinearOnly | kMix2)) == FLAG) { \
-    auto k = kern::solve32r<B, N, RMODE>;                                      \
+    auto k = kern::solve32r<B, N, RMODE, 0>;                                   \
     if (lds > 64 * 1024)                                                       \
       (void)hipFuncSetAttribute((const void *)k,                               \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,   \
@@ -2020,8 +2064,8 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
 #undef MF_VARIANT_LAUNCH
 #define MF_REG_LAUNCH(B, N, RMODE, FLAG)                                       \
   if (v.npt == N && v.block == B && (v.flags & kEllInRegs) &&                  \
-      (v.flags & (kLinearOnly | kMix2)) == FLAG) {                             \
-    auto k = kern::solve32r<B, N, RMODE>;                                      \
+      !(v.flags & kWaveRve) && (v.flags & (kLinearOnly | kMix2)) == FLAG) {    \
+    auto k = kern::solve32r<B, N, RMODE, 0>;                                   \
     if (lds > 64 * 1024)                                                       \
       (void)hipFuncSetAttribute((const void *)k,                               \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,   \
@@ -2032,6 +2076,29 @@ hipError_t launch_solve32(const Variant &v, const SolveArgs &a, int n_blocks,
 #define MF_REG_LAUNCH2(B, N)                                                   \
   MF_REG_LAUNCH(B, N, kern::RM_LIN, kLinearOnly)                               \
   MF_REG_LAUNCH(B, N, kern::RM_MIX2, kMix2)
+  // one RVE per wave: W RVEs per workgroup, the group's solves in
+  // ceil(n / W) workgroups, lds = W regions of the largest RVE's bytes
+#define MF_WAVE_LAUNCH(W, N, RMODE, FLAG)                                      \
+  if (v.npt == N && v.block == 64 * (W) && (v.flags & kWaveRve) &&             \
+      (v.flags & (kLinearOnly | kMix2)) == FLAG) {                             \
+    auto k = kern::solve32r<64 * (W), N, RMODE, W>;                            \
+    SolveArgs aw = a;                                                          \
+    aw.wave_lds = (int32_t)((lds + 15) & ~size_t(15));                        \
+    const size_t lw = (size_t)(W) * (size_t)aw.wave_lds;                       \
+    if (lw > kMaxLds) return hipErrorInvalidValue;                             \
+    if (lw > 64 * 1024)                                                        \
+      (void)hipFuncSetAttribute((const void *)k,                               \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)kMaxLds);                                 \
+    hipLaunchKernelGGL(k, dim3((n_blocks + (W) - 1) / (W)), dim3(64 * (W)), lw, s, aw); \
+    return hipGetLastError();                                                  \
+  }
+#define MF_WAVE_LAUNCH2(W, N)                                                  \
+  MF_WAVE_LAUNCH(W, N, kern::RM_LIN, kLinearOnly)                              \
+  MF_WAVE_LAUNCH(W, N, kern::RM_MIX2, kMix2)
+  MF_WAVE_LIST(MF_WAVE_LAUNCH2)
+#undef MF_WAVE_LAUNCH2
+#undef MF_WAVE_LAUNCH
   // only the instantiations that stay spill-free (the reaction-table mode
   // holds the table rows of every fiber: 256 and 512 threads only)
   MF_REG_LAUNCH(256, 1, kern::RM_TABLE, 0)

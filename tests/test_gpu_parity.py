@@ -590,6 +590,53 @@ def test_deterministic_repeat():
         assert np.array_equal(a.run(F), b.run(F))
 
 
+@pytest.mark.parametrize("nl", [0.0, 0.5])
+def test_one_rve_per_wave_agrees_with_one_rve_per_workgroup(nl):
+    """Small RVEs (120-180 nodes) on the one-RVE-per-wave register kernel,
+    one and four RVEs per workgroup (seven RVEs: the last workgroup of four
+    has a spare wave), against the one-RVE-per-workgroup register kernel.
+    Ramp mode: the same per-node arithmetic gives bitwise the same committed
+    state, the stress differs only by the order of its FP64 boundary sum,
+    and both are within the FP32c bar of the oracle.  Relax mode: every
+    solve converges and the two agree to 1e-3 (the residual's FP64 sum order
+    moves the stopping step, and at the default tolerance the stress still
+    drifts by ~1e-4 between nearby steps)."""
+    say(f"case: one RVE per wave, nonlinear={nl}")
+    for mode in (mf.MF_MODE_RAMP, mf.MF_MODE_RELAX):
+        extra = {} if mode == mf.MF_MODE_RAMP else dict(E=1e8, radius=5e-5)
+        nets = [mf.generate_network(seed=300 + i, n_nodes=n, nonlinear_fraction=nl, **extra)
+                for i, n in enumerate([120, 130, 140, 150, 160, 170, 180])]
+        F = strain_F(15, len(nets), 0.06 if mode == mf.MF_MODE_RAMP else 0.01)
+        common = dict(mode=mode, n_steps=1500, precision=mf.MF_PREC_FP32C,
+                      reaction_policy=mf.MF_REACT_PER_FIBER, chunks=1)
+        out = {}
+        for fk, bs in ((mf.MF_FP32_KERNEL_NODE, 256), (mf.MF_FP32_KERNEL_WAVE, 64),
+                       (mf.MF_FP32_KERNEL_WAVE, 256)):
+            g = mf.BatchedFiberRVEAnalysisExplicit(
+                nets, mf.default_params(fp32_kernel=fk, block_size=bs, **common))
+            _, plan = g.plan()
+            wave = [bool(int(f) & 4096) for f in plan[:, 2]]
+            assert all(wave) == (fk == mf.MF_FP32_KERNEL_WAVE) and any(wave) == all(wave), plan
+            s, st = g.run(F, return_status=True)
+            assert (st == mf.MF_STATUS_OK).all()
+            out[(fk, bs)] = (s, g.get_state()[1])
+        ref = out[(mf.MF_FP32_KERNEL_NODE, 256)]
+        w1, w4 = out[(mf.MF_FP32_KERNEL_WAVE, 64)], out[(mf.MF_FP32_KERNEL_WAVE, 256)]
+        assert np.array_equal(w1[0], w4[0]) and np.array_equal(w1[1], w4[1])
+        e_ref = rel_err(w1[0], ref[0])
+        say(f"  mode {mode}: wave vs workgroup stress {e_ref:.2e}, "
+            f"state bitwise {np.array_equal(w1[1], ref[1])}")
+        if mode == mf.MF_MODE_RAMP:
+            assert np.array_equal(w1[1], ref[1])
+            assert e_ref <= 1e-13
+            e_o = rel_err(w1[0], Oracle(nets, mf.default_params(
+                **dict(common, precision=mf.MF_PREC_FP64))).run(F))
+            say(f"  wave vs oracle {e_o:.2e}")
+            assert e_o <= FP32C_TOL
+        else:
+            assert e_ref <= 1e-3
+
+
 def test_register_kernel_slots_agree_bitwise():
     """The register-resident FP32c kernel with one or several free nodes per
     thread gives bitwise the same state (per-node arithmetic is the same),
