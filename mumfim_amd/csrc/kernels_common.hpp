@@ -32,6 +32,14 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
 }
+// a . b with the contraction spelt out (x product rounded once, y and z
+// fused): every kernel shape then rounds a fiber's stretch, a squared length
+// or a residual the same way, whatever the compiler's own contraction choice
+// in the surrounding code
+__device__ inline float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
+  return fmaf(az, bz, fmaf(ay, by, ax * bx));
+}
+
 __device__ inline double wave_min(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
         const float gz = q.z - uh[s][2];
         const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
         // l^2 - l0^2 = (d0 + d) . delta
-        const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+        const float sl = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
         const float e2 = sl * (il * il);
         const unsigned rid = (uint32_t)wv >> 24;
         float t;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const float x1 = (float)(X2.x - X1.x) * xq_inv + (p2.x - p1.x);
       const float x2 = (float)(X2.y - X1.y) * xq_inv + (p2.y - p1.y);
       const float x3 = (float)(X2.z - X1.z) * xq_inv + (p2.z - p1.z);
-      lmin = fminf(lmin, x1 * x1 + x2 * x2 + x3 * x3);
+      lmin = fminf(lmin, dot3f(x1, x2, x3, x1, x2, x3));
     }
     return (double)lmin;
   };
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       if (s * B + tid < Nf) {
         float f[3];
         node_force(s, cur, f);
-        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        if (refresh) rsum += (double)dot3f(f[0], f[1], f[2], f[0], f[1], f[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
           vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
       const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float sl = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
       const float e2 = sl * (il * il);
       const float t = lin_only ? lin_EA_half * il * e2
                                : react32_tab(rtab, (uint32_t)wv >> 24, e2, il);
@@ -411,9 +411,9 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     st[0] += xb[0] * g1;
     st[1] += xb[1] * g2;
     st[2] += xb[2] * g3;
-    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
-    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
-    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    st[3] += 0.5 * fma(xb[1], g3, xb[2] * g2);
+    st[4] += 0.5 * fma(xb[0], g3, xb[2] * g1);
+    st[5] += 0.5 * fma(xb[0], g2, xb[1] * g1);
     if (flags & kWriteX)
       for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
     if (flags & kWriteU)
@@ -771,8 +771,7 @@ void solve32r(SolveArgs a) {
         const float gz = q[j].z - uh[sl][2];
         const float dx = d0x[sl][j] + gx, dy = d0y[sl][j] + gy, dz = d0z[sl][j] + gz;
         // l^2 - l0^2 = (d0 + d) . delta
-        const float sq = (d0x[sl][j] + dx) * gx + (d0y[sl][j] + dy) * gy +
-                         (d0z[sl][j] + dz) * gz;
+        const float sq = dot3f(d0x[sl][j] + dx, d0y[sl][j] + dy, d0z[sl][j] + dz, gx, gy, gz);
         float t;
         if (RM == RM_LIN) {
           t = cf[sl][j] * sq;
@@ -831,7 +830,7 @@ void solve32r(SolveArgs a) {
       const float x1 = (float)(X[3 * n2] - X[3 * n1]) + (p2.x - p1.x);
       const float x2 = (float)(X[3 * n2 + 1] - X[3 * n1 + 1]) + (p2.y - p1.y);
       const float x3 = (float)(X[3 * n2 + 2] - X[3 * n1 + 2]) + (p2.z - p1.z);
-      lmin = fminf(lmin, x1 * x1 + x2 * x2 + x3 * x3);
+      lmin = fminf(lmin, dot3f(x1, x2, x3, x1, x2, x3));
     }
     return (double)lmin;
   };
@@ -916,7 +915,7 @@ void solve32r(SolveArgs a) {
           // the residual is |f|: linear forces are scaled back by m
           const float m = LIN ? imr[sl] : 1.0f;
           const float f0 = fint[sl][0] * m, f1 = fint[sl][1] * m, f2 = fint[sl][2] * m;
-          rsum += (double)(f0 * f0 + f1 * f1 + f2 * f2);
+          rsum += (double)dot3f(f0, f1, f2, f0, f1, f2);
         }
       }
     }
@@ -1018,7 +1017,7 @@ void solve32r(SolveArgs a) {
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
       const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float sq = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
       float t;
       if (LIN) {
         const double l0 = gl0[idx];
@@ -1035,9 +1034,9 @@ void solve32r(SolveArgs a) {
     st[0] += xb[0] * g1;
     st[1] += xb[1] * g2;
     st[2] += xb[2] * g3;
-    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
-    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
-    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    st[3] += 0.5 * fma(xb[1], g3, xb[2] * g2);
+    st[4] += 0.5 * fma(xb[0], g3, xb[2] * g1);
+    st[5] += 0.5 * fma(xb[0], g2, xb[1] * g1);
     if (flags & kWriteX)
       for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
     if (flags & kWriteU)
@@ -1264,7 +1263,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       const float gz = qb[s].z - qa[s].z;
       const float dx = d0x[s] + gx, dy = d0y[s] + gy, dz = d0z[s] + gz;
       // l^2 - l0^2 = (d0 + d) . delta
-      const float sq = (d0x[s] + dx) * gx + (d0y[s] + dy) * gy + (d0z[s] + dz) * gz;
+      const float sq = dot3f(d0x[s] + dx, d0y[s] + dy, d0z[s] + dz, gx, gy, gz);
       float t;
       if (RM == RM_LIN) {
         t = cf[s] * sq;
@@ -1279,7 +1278,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
         const float il = cf[s];
         t = react32_tab(rtab, (uint32_t)pa[s] >> 24, sq * (il * il), il);
       }
-      if (want_min && s * B + tid < M) lmin = fminf(lmin, dx * dx + dy * dy + dz * dz);
+      if (want_min && s * B + tid < M) lmin = fminf(lmin, dot3f(dx, dy, dz, dx, dy, dz));
       const float fx = dx * t, fy = dy * t, fz = dz * t;
       *reinterpret_cast<float4 *>(Fs + fa[s]) = make_float4(-fx, -fy, -fz, 0.0f);
       *reinterpret_cast<float4 *>(Fs + fb[s]) = make_float4(fx, fy, fz, 0.0f);
@@ -1412,7 +1411,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       if (s * B + tid < Nf) {
         float f[3];
         node_force(s, f);
-        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        if (refresh) rsum += (double)dot3f(f[0], f[1], f[2], f[0], f[1], f[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
           vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
@@ -1486,7 +1485,7 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
       const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float sq = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
       float t;
       if (LIN) {
         const double l0 = gl0[idx];
@@ -1503,9 +1502,9 @@ __global__ __launch_bounds__(B) void solve32f(SolveArgs a) {
     st[0] += xb[0] * g1;
     st[1] += xb[1] * g2;
     st[2] += xb[2] * g3;
-    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
-    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
-    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    st[3] += 0.5 * fma(xb[1], g3, xb[2] * g2);
+    st[4] += 0.5 * fma(xb[0], g3, xb[2] * g1);
+    st[5] += 0.5 * fma(xb[0], g2, xb[1] * g1);
     if (flags & kWriteX)
       for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
     if (flags & kWriteU)
@@ -1718,7 +1717,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       const float gx = qb.x - qa.x, gy = qb.y - qa.y, gz = qb.z - qa.z;
       const float dx = px + gx, dy = py + gy, dz = pz + gz;
       // l^2 - l0^2 = (d0 + d) . delta
-      const float sq = (px + dx) * gx + (py + dy) * gy + (pz + dz) * gz;
+      const float sq = dot3f(px + dx, py + dy, pz + dz, gx, gy, gz);
       float t;
       if (RM == RM_LIN) {
         t = pc * sq;
@@ -1732,7 +1731,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       }
       const int e = s * B + tid;
       if (e < M) {
-        if (want_min) lmin = fminf(lmin, dx * dx + dy * dy + dz * dz);
+        if (want_min) lmin = fminf(lmin, dot3f(dx, dy, dz, dx, dy, dz));
         ffr[e] = F3{dx * t, dy * t, dz * t};
       }
     }
@@ -1886,7 +1885,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       if (s * B + tid < Nf) {
         float f[3];
         node_force(s, f);
-        if (refresh) rsum += (double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        if (refresh) rsum += (double)dot3f(f[0], f[1], f[2], f[0], f[1], f[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
           vr[s][d] = vr[s][d] + hk * (-f[d] * imr[s] - eta * vr[s][d]);
@@ -1960,7 +1959,7 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
       const float gy = (qh.y - hb.y) + (ql.y - lb.y);
       const float gz = (qh.z - hb.z) + (ql.z - lb.z);
       const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-      const float sq = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
+      const float sq = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
       float t;
       if (LIN) {
         const double l0 = gl0[idx];
@@ -1977,9 +1976,9 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
     st[0] += xb[0] * g1;
     st[1] += xb[1] * g2;
     st[2] += xb[2] * g3;
-    st[3] += 0.5 * (xb[1] * g3 + xb[2] * g2);
-    st[4] += 0.5 * (xb[0] * g3 + xb[2] * g1);
-    st[5] += 0.5 * (xb[0] * g2 + xb[1] * g1);
+    st[3] += 0.5 * fma(xb[1], g3, xb[2] * g2);
+    st[4] += 0.5 * fma(xb[0], g3, xb[2] * g1);
+    st[5] += 0.5 * fma(xb[0], g2, xb[1] * g1);
     if (flags & kWriteX)
       for (int d = 0; d < 3; ++d) a.x_out[3 * gr + d] = xb[d];
     if (flags & kWriteU)
