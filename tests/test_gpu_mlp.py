@@ -9,6 +9,7 @@ outputs and Jacobians within 2e-3 of their largest magnitude, gradients
 within 2e-3 in relative L2 norm per parameter block.  Against the unrounded
 float64 network (the mathematics) the bar is 5e-2.
 """
+import math
 import sys
 
 import numpy as np
@@ -186,3 +187,67 @@ def test_training_fits_a_smooth_map_and_is_deterministic():
     rms = np.sqrt(((Yp - Yt) ** 2).mean()) / Yt.std()
     say(f"  held-out relative rms {rms:.3e}, mean spread {S.mean():.3e}")
     assert rms < 0.1
+
+
+def _splitmix_perm(seed, stream, n):
+    """The fit's per-epoch, per-member shuffle: Fisher-Yates driven by
+    SplitMix64(seed, stream) (the library's generator)."""
+    M = (1 << 64) - 1
+
+    def mix(z):
+        z = (z + 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    s = mix(seed ^ mix(stream))
+    p = list(range(n))
+    for i in range(n - 1, 0, -1):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        j = z % (i + 1)
+        p[i], p[j] = p[j], p[i]
+    return np.array(p, np.int32)
+
+
+def test_fit_equals_its_steps():
+    """fit over three epochs gives bitwise the weights of the same Adam steps
+    launched one by one through train_step, with the fit's per-epoch
+    SplitMix64 shuffles and its cosine learning-rate schedule restated here:
+    pins the shuffle, the schedule and the bias-correction step count."""
+    say("case: fit vs its steps")
+    rng = np.random.default_rng(27)
+    n, B, epochs, K, seed = 2048, 256, 3, 2, 31
+    lr0, lr1 = 2e-3, 1e-4
+    X = rng.normal(0, 1, (n, 10)).astype(np.float32)
+    Y = _smooth_target(X, 28)
+    Xd, Yd = mf.DeviceArray.from_host(X), mf.DeviceArray.from_host(Y)
+    a = mf.SurrogateMLP(n_members=K, seed=29)
+    b = mf.SurrogateMLP(n_members=K, seed=29)
+    for m in (a, b):
+        m.set_normalization(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
+    tp = mf.default_train_params()
+    assert (tp.beta1, tp.beta2) == (np.float32(0.9), np.float32(0.999))
+    hist = a.fit(Xd, Yd, n, epochs=epochs, batch=B, lr=lr0, lr_final=lr1, seed=seed)
+    steps = n // B
+    total = steps * epochs
+    for ep in range(epochs):
+        perms = [_splitmix_perm(seed, ep * K + k, n) for k in range(K)]
+        for st in range(steps):
+            it = ep * steps + st
+            frac = it / (total - 1)
+            # the schedule as the library evaluates it: lr - lr_final in FP32,
+            # the rest in double, rounded to FP32
+            l1 = float(np.float32(lr1))
+            dl = float(np.float32(lr0) - np.float32(lr1))
+            lr = float(np.float32(l1 + 0.5 * dl * (1.0 + math.cos(math.pi * frac))))
+            idx = np.concatenate([p[st * B:(st + 1) * B] for p in perms]).astype(np.int32)
+            b.train_step(Xd, Yd, n, B, lr, idx_dev=mf.DeviceArray.from_host(idx),
+                         return_loss=False)
+    for k in range(K):
+        assert np.array_equal(a.get_weights(k), b.get_weights(k)), k
+    say(f"  epoch losses {hist.mean(1).round(5).tolist()}")
+    assert hist[-1].mean() < hist[0].mean()
