@@ -154,7 +154,7 @@ typedef struct mf_solver_params {
  * evaluate each fiber once and let nodes sum the published end forces (fewer
  * arithmetic operations, more LDS traffic; RVEs up to ~1800 nodes).  WAVE
  * restricts the node-centric family to its one-RVE-per-wave shapes (each
- * wave solves its own RVE of up to ~200 nodes with no workgroup barrier;
+ * wave solves its own RVE of up to ~180 nodes with no workgroup barrier;
  * block_size 64 W packs W RVEs per workgroup).  AUTO lets the planner and
  * autotuner choose per RVE. */
 #define MF_FP32_KERNEL_AUTO 0
