@@ -274,8 +274,10 @@ int mf_batch_rve_sizes(const mf_batch *b, int32_t *sizes);
  * bytes.  Flags: 1 incidence lists in LDS, 2 double-buffered positions,
  * 4 incidence lists in registers, 8 linear-only, 16 energy diagnostics,
  * 32 one linear and one nonlinear reaction, 64 fiber-parallel, 128
- * fiber-streamed (bits 8-11: fibers per thread), 8192 FP64 with the state in
- * global memory (RVEs no other variant holds, in either precision), 16384
+ * fiber-streamed (bits 8-11: fibers per thread), 4096 one RVE per wave
+ * (the first entry then counts the workgroup's threads, 64 per RVE), 8192
+ * FP64 with the state in global memory (RVEs no other variant holds, in
+ * either precision), 16384
  * FP64 evaluating each fiber once (bitwise the node-centric FP64 result);
  * bits 24-28: step chunks per solve of run() when more than one
  * (params.chunks).
