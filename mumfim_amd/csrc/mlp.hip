@@ -56,9 +56,16 @@ __device__ __forceinline__ uint16_t bf(float x) {
 __device__ __forceinline__ float unbf(uint16_t b) {
   return __uint_as_float(static_cast<uint32_t>(b) << 16);
 }
+// two floats to BF16 (round to nearest even) in one 32-bit word, low half
+// first: one v_cvt_pk_bf16_f32 (converted one at a time, each value took a
+// conversion and the pair a shift and an or)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
 __device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
-  return make_uint2(static_cast<uint32_t>(bf(a)) | (static_cast<uint32_t>(bf(b)) << 16),
-                    static_cast<uint32_t>(bf(c)) | (static_cast<uint32_t>(bf(d)) << 16));
+  return make_uint2(pack2(a, b), pack2(c, d));
 }
 // silu(z) = z * sigmoid(z); ds = silu'(z) = sg * (1 + z * (1 - sg))
 __device__ __forceinline__ float silu(float z, float &ds) {
