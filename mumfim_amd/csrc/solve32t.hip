@@ -134,7 +134,11 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
   const double inv_S = 1.0 / a.n_steps;
 
   float uh[NPT][3], ul[NPT][3], vr[NPT][3], du[NPT][3], dv[NPT][3], imr[NPT];
-  int Dr[NPT], sbase[NPT];
+  // slice width and base (wave-uniform); Dn: with one linear and one
+  // nonlinear reaction, entries [0, Dn) hold the wave's nonlinear fibers
+  // (nodes list them first) and later entries are linear on every lane, so
+  // the wave skips the toe law and its derivative there
+  int Dr[NPT], sbase[NPT], Dn[NPT];
   // ramp mode: u kept, boundary increment (F - I) x and its derivative G x
   // set aside; derivatives start at zero (the start state does not depend
   // on the perturbation)
@@ -183,6 +187,14 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
     const int k = r / kWave;
     Dr[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_D[R.slice_off + k] : 0);
     sbase[s] = __builtin_amdgcn_readfirstlane(r < Nf ? a.slice_base[R.slice_off + k] : 0);
+    int cnt = 0;
+    if (mix2 && r < Nf) {
+      const int dg = a.deg[nb + r];
+      while (cnt < dg && (int)((uint32_t)gw[sbase[s] + lane + cnt * kWave] >> 24) == nl_rid) ++cnt;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o, kWave));
+    Dn[s] = __builtin_amdgcn_readfirstlane(cnt);
   }
   for (int r = NPT * B + tid; r < N; r += B) {
     double u[3];
@@ -232,8 +244,13 @@ __global__ __launch_bounds__(B) void solve32t(SolveArgs a) {
         const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
         const float sl = (ex + dx) * gx + (ey + dy) * gy + (ez + dz) * gz;
         const float il2 = il * il;
-        float dt;
-        const float t = react_d(wv, sl * il2, il, dt);
+        float dt, t;
+        if (mix2 && j >= Dn[s]) {  // linear on every lane of the wave
+          dt = ea_half_lin * il;
+          t = dt * (sl * il2);
+        } else {
+          t = react_d(wv, sl * il2, il, dt);
+        }
         // d(l^2 - l0^2) = 2 d . dd
         const float ddx = dq.x - du[s][0], ddy = dq.y - du[s][1], ddz = dq.z - du[s][2];
         const float de2 = 2.0f * (dx * ddx + dy * ddy + dz * ddz) * il2;
