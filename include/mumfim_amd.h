@@ -237,10 +237,16 @@ int mf_batch_last_solve_ms(mf_batch *b, double *ms, int32_t *n_launches);
  * kernel variant that fits (threads per solve, nodes per thread, incidence
  * lists in registers / LDS / HBM, one or two position buffers) is timed on
  * ramp solves of `steps` steps of the bucket's RVEs, and the fastest is kept
- * for that bucket.  With cache_path, buckets found in the file are taken
- * from it and newly timed ones are appended (text: prec lin regok lanes/64
- * nodes/256 block npt flags us_per_solve).  n_timed (may be NULL) counts
- * the timed candidates.  The batch state is left untouched. */
+ * for that bucket.  The tuned buckets are kept only if the whole batch then
+ * runs faster than with the plan they replace.  For ramp-mode FP32c batches
+ * the step chunks per solve of the register-kernel groups (the planner's
+ * count, or 1-4) are then timed on the batch's own run path and the fastest
+ * is kept (params.chunks, when set, still forces the count; the chunk
+ * choice is not cached).  With cache_path, buckets found in the file are
+ * taken from it and newly timed ones are appended (text: prec lin regok
+ * lanes/64 nodes/256 block npt flags us_per_solve).  n_timed (may be NULL)
+ * counts the timed kernel-variant candidates.  The batch state is left
+ * untouched. */
 int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps,
                       const char *cache_path, int32_t *n_timed);
 /* hipGraph mode for mf_batch_run_device_async: the launch sequence of a run

@@ -460,7 +460,9 @@ class BatchedFiberRVEAnalysisExplicit:
 
     def autotune(self, precision=None, steps=200, cache_path=None):
         """Times the fitting kernel variants per RVE bucket and keeps the
-        fastest; returns the number of timed candidates."""
+        fastest (if the whole batch is then faster), then, for ramp-mode
+        FP32c batches, the step chunks per solve; returns the number of
+        timed kernel-variant candidates."""
         prec = self.params.precision if precision is None else precision
         n = C.c_int32()
         check(lib().mf_batch_autotune(self.h, int(prec), int(steps),

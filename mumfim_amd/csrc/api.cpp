@@ -211,6 +211,7 @@ struct mf_batch {
   DevBuf<double> solve_G, dsigma;
   int no_reg_variants = 0;  // planner restricted to the LDS/global-list kernels
   std::map<TuneKey, int> tuned;  // autotuned variant per bucket
+  int tuned_chunks = 0;           // autotuned step chunks per solve (0: the model's)
   int plan_version = 0;          // bumped whenever the launch tables change
   // hipGraph replay of run_device_fast (mf_batch_set_graph)
   struct Graph {
@@ -581,12 +582,13 @@ int64_t group_gstride(const Variant &v, int N) {
 // c chunks per solve take ceil(n c / S) rounds of 1/c the length.  The
 // count minimising that (plus 1% per extra chunk for the state round trip)
 // is chosen, and only when n >= 2 S, so a chunk's predecessor has finished
-// long before it starts.  params.chunks > 0 forces the count.
+// long before it starts.  params.chunks > 0 forces the count, else the
+// autotuner's measured count when it set one.
 int plan_chunks(const mf_batch *b, const Variant &v, size_t lds, int n) {
   if (b->p.mode != MF_MODE_RAMP || v.prec != MF_PREC_FP32C || !(v.flags & kEllInRegs) ||
       (v.flags & kWaveRve))
     return 1;
-  const int force = b->p.chunks;
+  const int force = b->p.chunks > 0 ? b->p.chunks : b->tuned_chunks;
   if (force > 0) return std::max(1, std::min(force, b->p.n_steps));
   const int occ = chunk_occupancy(v, lds);
   int cus = 0;
@@ -1671,6 +1673,46 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
   int e = make_plan(b, prec);
   if (e != MF_OK) return e;
   if ((e = build_identity_tables(b)) != MF_OK) return e;
+  // Step chunks per solve (ramp FP32c runs, register-kernel groups): the
+  // model's count and 1-4 forced are timed on the batch's own run path
+  // (trial runs of 4x the candidate steps, which leave the state untouched)
+  // and the fastest is kept; params.chunks, when set, still wins.
+  if (prec == MF_PREC_FP32C && p0.precision == MF_PREC_FP32C && p0.mode == MF_MODE_RAMP &&
+      p0.chunks == 0) {
+    bool chunkable = false;
+    for (size_t gi = 0; gi < b->id_groups[prec].size(); ++gi) {
+      const Variant &v = vs[b->id_groups[prec][gi].first];
+      chunkable = chunkable || ((v.flags & kEllInRegs) && !(v.flags & kWaveRve));
+    }
+    if (chunkable) {
+      const int tc0 = b->tuned_chunks;
+      b->p.n_steps = 4 * steps;
+      const double Fi[9] = {1.01, 0, 0, 0, 1, 0, 0, 0, 1};
+      MF_HIP(b->F_in.ensure(9));
+      MF_HIP(hipMemcpyAsync(b->F_in.p, Fi, sizeof(Fi), hipMemcpyHostToDevice, b->stream));
+      int best_c = tc0;
+      double best_ms = 0.0;
+      for (int c = 0; c <= 4 && e == MF_OK; ++c) {
+        b->tuned_chunks = c;
+        if ((e = build_identity_tables(b)) != MF_OK) break;
+        double ms = 0.0;
+        for (int rep = 0; rep < 2 && e == MF_OK; ++rep) {  // warm-up, then timed
+          e = run_device_fast(b, b->F_in.p, 1, 0, nullptr, nullptr);
+          if (e == MF_OK) e = finish_timing(b);
+          ms = b->last_ms;
+        }
+        if (e == MF_OK && (c == 0 || ms < best_ms)) {
+          best_c = c;
+          best_ms = ms;
+        }
+      }
+      b->p = p0;
+      b->tuned_chunks = e == MF_OK ? best_c : tc0;
+      const int e2 = build_identity_tables(b);
+      if (e != MF_OK) return e;
+      if ((e = e2) != MF_OK) return e;
+    }
+  }
   if (cache_path && !added.str().empty()) {
     const bool fresh = !std::ifstream(cache_path).good();
     std::ofstream f(cache_path, std::ios::app);

@@ -185,3 +185,14 @@ def test_planner_chunks_the_500_node_batch():
     say(f"  kernel ms: chunked {ms_chunked:.2f}, one piece {ms_plain:.2f}")
     assert (sta == 0).all()
     assert np.array_equal(sa, s1)
+    # the autotuner times the chunk counts on the run path and keeps one
+    # (a count of 1-4 per register-kernel solve); the bits do not move
+    gt = mf.BatchedFiberRVEAnalysisExplicit(params=mf.default_params(**kw), generated=(gen, R))
+    gt.autotune(steps=100)
+    nch_t = gt.plan()[1][:, 2] >> 24
+    st_, stt = gt.run(F, update_coords=False, return_status=True)
+    say(f"  tuned chunks per solve: {np.unique(nch_t, return_counts=True)}, "
+        f"kernel ms {gt.last_solve_ms()[0]:.2f}")
+    gt.close()
+    assert nch_t.max() <= 4 and (stt == 0).all()
+    assert np.array_equal(st_, s1)
