@@ -829,7 +829,7 @@ int mlp_predict(mf_mlp *m, int member, const float *X, int64_t n, float *Y, floa
     hipLaunchKernelGGL((infer_kernel<Q, true>), dim3((unsigned)((n + 16 * Q - 1) / (16 * Q))),
                        dim3(256), lds, s, a);
   } else {
-    constexpr int Q = 8, P = 8;
+    constexpr int Q = 4, P = 4;
     const size_t lds = (size_t)P * 16 * kActLd * sizeof(uint16_t);
     hipLaunchKernelGGL((infer_kernel<Q, false>), dim3((unsigned)((n + 16 * Q - 1) / (16 * Q))),
                        dim3(256), lds, s, a);
@@ -905,8 +905,8 @@ int mf_mlp_create(int32_t n_members, uint64_t seed, mf_mlp **out) {
   }
   static bool attr_done = false;
   if (!attr_done) {
-    const int big = 8 * 16 * kActLd * (int)sizeof(uint16_t);
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&infer_kernel<8, false>),
+    const int big = 4 * 16 * kActLd * (int)sizeof(uint16_t);
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&infer_kernel<4, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, big) != hipSuccess ||
         hipFuncSetAttribute(reinterpret_cast<const void *>(&infer_kernel<1, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize,
