@@ -552,6 +552,42 @@ def bench_tangentlinear(args):
                       "dtype": "f32"}), flush=True)
 
 
+def bench_sizes(args):
+    """FP32c and FP64 throughput per RVE size: batches of 4096 equal-size RVEs
+    (N nodes, all-linear or half the fibers on the toe law, F = I + sym(U),
+    U ~ U(-0.05, 0.05), 5000 ramp steps), one JSON line per (N, reactions)
+    with the launch groups the planner chose.  FP64 is timed on 512 RVEs."""
+    import mumfim_amd as mf
+    R, S = 4096, args.explicit_steps
+    rows = []
+    for nl in (0.0, 0.5):
+        for n in (150, 300, 500, 1000, 2000, 3000, 4000):
+            gen = mf.default_gen_params(seed=300001 + n, n_nodes=n, nonlinear_fraction=nl)
+            F = strain_F(5, R, 0.05)
+            row = {"config": "sizes", "n_nodes": n, "nonlinear_fraction": nl, "n_rve": R,
+                   "explicit_steps": S}
+            for name, prec, nr in (("fp32c", mf.MF_PREC_FP32C, R), ("fp64", mf.MF_PREC_FP64, 512)):
+                p = mf.default_params(mode=mf.MF_MODE_RAMP, n_steps=S, precision=prec,
+                                      reaction_policy=mf.MF_REACT_PER_FIBER)
+                g = mf.BatchedFiberRVEAnalysisExplicit(params=p, generated=(gen, nr))
+                F_dev = mf.DeviceArray.from_host(F[:nr].reshape(nr, 9))
+                sig = mf.DeviceArray((nr, 6), np.float64)
+                st = mf.DeviceArray((nr,), np.int32)
+                wall, kms = time_solver(g, F_dev, sig, args.steps, args.warmup, st)
+                _, plan = g.plan()
+                shapes = sorted({(int(b), int(t), int(f) & 0xFFFFFF) for b, t, f, _ in plan})
+                row[name] = {"solves_per_s": round(nr * args.steps / wall, 1),
+                             "ms_per_batch": round(1e3 * wall / args.steps, 2), "n_rve": nr,
+                             "status_ok": bool((st.numpy() == mf.MF_STATUS_OK).all()),
+                             "plan": [list(x) for x in shapes]}
+                g.close()
+            log(f"[sizes] N={n} nonlinear={nl}: fp32c {row['fp32c']['solves_per_s']}/s, "
+                f"fp64 {row['fp64']['solves_per_s']}/s")
+            rows.append(row)
+            print(json.dumps(row), flush=True)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -567,12 +603,13 @@ def main():
     ap.add_argument("--only", choices=["fp32c", "fp64"], default=None,
                     help="time one solver only (profiling runs)")
     ap.add_argument("--config", choices=["batch500", "mixed16k", "tangent1k2k", "tangentlinear",
-                                         "surrogate"], default="batch500",
+                                         "surrogate", "sizes"], default="batch500",
                     help="batch500 (default): 4096 x 500 nodes; mixed16k: 16384 RVEs of 200-4000 "
                          "nodes, mixed reactions, 10000 steps; tangent1k2k: 4096 RVEs of "
                          "1000-2000 nodes, base + 12 perturbed solves per RVE; tangentlinear: the "
                          "same RVEs, FP32 tangent-linear only (6 solves per RVE); surrogate: learned "
-                         "surrogate trained on 1M solves, inference on 1M points")
+                         "surrogate trained on 1M solves, inference on 1M points; sizes: FP32c and "
+                         "FP64 solves/s of 4096 equal RVEs of 150-4000 nodes, linear and toe law")
     ap.add_argument("--surrogate-networks", type=int, default=65536,
                     help="surrogate: networks in the training data (16 strains each)")
     ap.add_argument("--surrogate-epochs", type=int, default=30)
@@ -595,6 +632,8 @@ def main():
         return bench_surrogate(args)
     if args.config == "tangentlinear":
         return bench_tangentlinear(args)
+    if args.config == "sizes":
+        return bench_sizes(args)
     R, S = args.n_rve, args.explicit_steps
     gen = mf.default_gen_params(seed=1, n_nodes=WORKLOAD["n_nodes"])
     F = strain_F(2, R, WORKLOAD["F_amp"])
