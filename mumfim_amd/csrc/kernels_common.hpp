@@ -351,9 +351,11 @@ struct Lin64 {
 
 // Fiber-streamed FP32c variants (large RVEs): (threads, fibers per thread,
 // free nodes per thread, reaction mode flag kLinearOnly|kMix2).  The widest
-// ones spill a few registers in the step loop (7 toe-law or 8 linear fibers
-// per thread) and still beat the LDS-list kernel; 8 toe-law fibers per
-// thread did not.
+// 1024-thread ones spill a few registers in the step loop (7 toe-law or 8
+// linear fibers per thread) and still beat the LDS-list kernel; 8 toe-law
+// fibers per thread did not.  At 768 threads (168 VGPRs) 7 fibers and 3
+// nodes per thread fit without spills and run 1900-2100-node toe-law RVEs
+// 2% faster than 8 and 4.
 // One-RVE-per-wave FP32c register variants: (RVEs per workgroup, free-node
 // slots per thread), each in the linear and one-linear-one-nonlinear modes
 // (four slots per thread spill registers).
@@ -366,6 +368,7 @@ struct Lin64 {
   X(1024, 7, 3, kLinearOnly) X(1024, 7, 3, kMix2)                            \
   X(1024, 8, 3, kLinearOnly)                                                 \
   X(768, 6, 3, kLinearOnly) X(768, 6, 3, kMix2)                              \
+  X(768, 7, 3, kLinearOnly) X(768, 7, 3, kMix2)                              \
   X(768, 8, 4, kLinearOnly) X(768, 8, 4, kMix2)                              \
   X(768, 10, 4, kLinearOnly) X(768, 10, 4, kMix2)
 
