@@ -159,7 +159,7 @@ def test_relax_multi_rve_broadcast_trial(net27):
 
 # ---------------------------------------------------------------- ramp mode
 def test_ramp_500_nodes_5000_steps_fp64_and_fp32c():
-    """Config 1: one RVE, 500 nodes, F = diag(1.05,1,1) ramped over 5000 steps."""
+    """One RVE of 500 nodes, F = diag(1.05,1,1) ramped over 5000 steps."""
     say("case: one 500-node RVE, ramp, 5000 steps")
     net = mf.generate_network(seed=1, n_nodes=500)
     F = np.diag([1.05, 1.0, 1.0])
@@ -177,7 +177,8 @@ def test_ramp_500_nodes_5000_steps_fp64_and_fp32c():
 
 
 def test_batch_500_nodes_subset_fp64_and_fp32c():
-    """Config 2 generator and F distribution on 32 RVEs (seeds 1..32)."""
+    """The 4096 x 500-node batch's generator and F distribution on 32 RVEs
+    (seeds 1..32)."""
     R = 32
     say(f"case: 500-node batch subset R={R}")
     nets = [mf.generate_network(seed=1 + r, n_nodes=500) for r in range(R)]
@@ -284,7 +285,8 @@ def test_full_size_mixed_batch_16384_fp32c():
 
 
 def test_uneven_mixed_reactions():
-    """Config 3 shape on a few RVEs: log-uniform sizes, half nonlinear fibers."""
+    """The mixed 200-4000-node batch on a few RVEs: spread sizes, half the
+    fibers on the toe law."""
     sizes = [200, 430, 910, 1650, 2700, 4000]
     say(f"case: uneven mixed sizes {sizes}")
     nets = [mf.generate_network(seed=100 + i, n_nodes=n, nonlinear_fraction=0.5)
@@ -320,7 +322,7 @@ def test_kernel_variants_agree_bitwise(block):
 
 @pytest.mark.parametrize("scheme", [mf.MF_TANGENT_FORWARD, mf.MF_TANGENT_CENTRAL])
 def test_ramp_tangent_fp64(scheme):
-    """Config 4 shape: base + 6 forward or 12 central perturbations
+    """The 1000-2000-node tangent batch's shape: base + 6 forward or 12 central perturbations
     (h = 1e-6) in one launch (the fused run + tangent), then the same tangent
     from a separate run and tangent call."""
     R = 4
