@@ -614,7 +614,7 @@ def main():
                     help="surrogate: networks in the training data (16 strains each)")
     ap.add_argument("--surrogate-epochs", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="mixed16k: keep the heuristic plan")
-    ap.add_argument("--tune-steps", type=int, default=100,
+    ap.add_argument("--tune-steps", type=int, default=500,
                     help="mixed16k: ramp steps per autotuner candidate")
     args = ap.parse_args()
     if args.gpus != 1:
