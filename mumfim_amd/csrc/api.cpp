@@ -1629,9 +1629,14 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
   }
   // Whole-batch check (also for buckets read from the cache): buckets timed
   // alone can mispredict the batch, whose launch groups run side by side;
-  // keep the tuned buckets only if the full batch (one solve per RVE, 4x the
-  // candidate steps) runs faster with them than with the plan they replace.
-  // The per-bucket timings go to the cache either way.
+  // keep the tuned buckets only if the full batch (one solve per RVE) runs
+  // faster with them than with the plan they replace.  The check runs 4x
+  // the candidate steps, and at least the batch's own ramp length up to 2000
+  // steps: with 100-step candidates a 400-step check kept a plan that ran 1%
+  // slower at 10000 steps.  The per-bucket timings go to the cache either
+  // way.
+  const int check_steps =
+      std::max(4 * steps, p0.mode == MF_MODE_RAMP ? std::min(p0.n_steps, 2000) : 0);
   if (err == MF_OK && b->tuned != tuned0) {
     SolveSet S;
     S.prec = prec;
@@ -1644,7 +1649,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
       S.F[9 * r + 4] = 1.0;
       S.F[9 * r + 8] = 1.0;
     }
-    b->p.n_steps = 4 * steps;
+    b->p.n_steps = check_steps;
     auto batch_ms = [&](double &ms) {
       int e = make_plan(b, prec);
       const int cidx = b->cur;
@@ -1675,7 +1680,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
   if ((e = build_identity_tables(b)) != MF_OK) return e;
   // Step chunks per solve (ramp FP32c runs, register-kernel groups): the
   // model's count and 1-4 forced are timed on the batch's own run path
-  // (trial runs of 4x the candidate steps, which leave the state untouched)
+  // (trial runs of the check's length, which leave the state untouched)
   // and the fastest is kept; params.chunks, when set, still wins.
   if (prec == MF_PREC_FP32C && p0.precision == MF_PREC_FP32C && p0.mode == MF_MODE_RAMP &&
       p0.chunks == 0) {
@@ -1686,7 +1691,7 @@ int mf_batch_autotune(mf_batch *b, int32_t precision, int32_t steps, const char 
     }
     if (chunkable) {
       const int tc0 = b->tuned_chunks;
-      b->p.n_steps = 4 * steps;
+      b->p.n_steps = check_steps;
       const double Fi[9] = {1.01, 0, 0, 0, 1, 0, 0, 0, 1};
       MF_HIP(b->F_in.ensure(9));
       MF_HIP(hipMemcpyAsync(b->F_in.p, Fi, sizeof(Fi), hipMemcpyHostToDevice, b->stream));
