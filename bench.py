@@ -444,8 +444,9 @@ def mixed16k_batch(mf, prec):
 
 
 def extra_mixed16k():
-    """Mixed-size batch: FP32c (1 warm-up + 2 timed batches, heuristic plan)
-    and FP64 (1 timed batch) on the whole batch; error of every RVE."""
+    """Mixed-size batch: FP32c (1 warm-up + 2 timed batches, heuristic plan,
+    then again after mf_batch_autotune with 500-step candidates) and FP64
+    (1 timed batch) on the whole batch; error of every RVE (heuristic plan)."""
     import mumfim_amd as mf
     t = time.perf_counter()
     g, F, sizes, S = mixed16k_batch(mf, mf.MF_PREC_FP32C)
@@ -458,6 +459,11 @@ def extra_mixed16k():
     s32, ok32 = sig.numpy(), bool((st.numpy() == 0).all())
     b32 = algorithmic_bytes(g, "fp32c") * S
     groups = g.plan()[0]
+    t = time.perf_counter()
+    g.autotune(mf.MF_PREC_FP32C, steps=500)
+    tune_s = time.perf_counter() - t
+    wall_t, _ = time_solver(g, F_dev, sig, 2, 1, st)
+    ok_t = bool((st.numpy() == 0).all())
     g.set_params(precision=mf.MF_PREC_FP64)
     w64, k64 = time_solver(g, F_dev, sig, 1, 0, st)
     s64, ok64 = sig.numpy(), bool((st.numpy() == 0).all())
@@ -468,6 +474,9 @@ def extra_mixed16k():
                         f"50% toe-law fibers, F=I+sym(U) U~U(-0.1,0.1), {S} ramp steps",
             "fp32c_solves_per_s": round(2 * R / wall, 1), "fp32c_ms_per_batch": round(500 * wall, 1),
             "fp32c_kernel_ms": round(k32, 1), "launch_groups": groups,
+            "fp32c_autotuned": {"solves_per_s": round(2 * R / wall_t, 1),
+                                "ms_per_batch": round(500 * wall_t, 1), "tune_s": round(tune_s, 1),
+                                "steps_per_candidate": 500, "status_all_ok": ok_t},
             "fp64_solves_per_s": round(R / w64, 1), "fp64_ms_per_batch": round(1e3 * w64, 1),
             "speedup_vs_fp64": round(w64 / (wall / 2), 3),
             "rel_stress_err_vs_fp64": {"n": R, "max": float(e.max()),
