@@ -11,6 +11,14 @@ namespace kern {
 // RM_LIN: the RVE has one linear reaction; RM_MIX2: at most one linear and
 // one nonlinear reaction, toe range within the expm1 polynomial's bound.
 enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
+// Law of a run of list entries in the LDS-list kernel: the one linear
+// reaction, the nonlinear-first entries of a one-linear-one-nonlinear RVE
+// (either law per lane), its later entries (linear on every lane), or the
+// packed table.
+enum { kLawLin = 0, kLawMixNl = 1, kLawMixLin = 2, kLawTab = 3 };
+// Fourth word of a raw buffer resource on gfx9-family GPUs: 32-bit data
+// format (a zero format makes untyped buffer loads return 0).
+constexpr int kBufRsrcWord3 = 0x00020000;
 
 // Pointers into LDS (address space 3): byte addresses within the
 // workgroup's LDS allocation.
@@ -74,6 +82,13 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
   }
   const int32_t *Ew = kEll ? ws : gw;
   const float *Eil = kEll ? ils : gil;
+  // HBM lists as raw buffers of the RVE's free-node entries (loads past the
+  // end return 0; every load here stays within the slices)
+  const int list_bytes = R.ell_free_entries * (int)sizeof(int32_t);
+  const __amdgpu_buffer_rsrc_t rsw =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t *>(gw), 0, list_bytes, kBufRsrcWord3);
+  const __amdgpu_buffer_rsrc_t rsil =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(gil), 0, list_bytes, kBufRsrcWord3);
   if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
   int nl_rid = -1;
   float ea_half_lin = 0.0f;
@@ -177,48 +192,77 @@ __global__ __launch_bounds__(B) void solve32c(SolveArgs a) {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int D = Dr[s];
     if (D > 0) {
-      int32_t wv_n = Ew[base], wv_nn = 0;
-      float il_n = Eil[base], il_nn = 0.0f;
-      if (D > 1) {
-        wv_nn = Ew[base + kWave];
-        il_nn = Eil[base + kWave];
-      }
-      for (int j = 0; j < D; ++j) {
-        const int32_t wv = wv_n;
-        const float il = il_n;
-        wv_n = wv_nn;
-        il_n = il_nn;
-        if (j + 2 < D) {
-          wv_nn = Ew[base + (j + 2) * kWave];
-          il_nn = Eil[base + (j + 2) * kWave];
+      // entry jj of the lane's list: from LDS, or from HBM through the
+      // buffer resources, whose lane offset is fixed for the slot and whose
+      // entry offset is a scalar (no per-entry address arithmetic)
+      const int vo = base * (int)sizeof(int32_t);
+      auto entry = [&](int jj, int32_t &wv, float &il) __attribute__((always_inline)) {
+        if constexpr (kEll) {
+          wv = Ew[base + jj * kWave];
+          il = Eil[base + jj * kWave];
+        } else {
+          const int so = jj * (kWave * (int)sizeof(int32_t));
+          wv = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rsw, vo, so, 0);
+          il = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsil, vo, so, 0));
         }
-        const int w = wv & 0xFFFFFF;
-        const float4 q = ld16(P + w);
-        const int4 xq = ld16(Xs + w);
-        // reference vector: exact integer difference, one rounding to FP32
-        const float ex = (float)(xq.x - Xv.x) * xq_inv;
-        const float ey = (float)(xq.y - Xv.y) * xq_inv;
-        const float ez = (float)(xq.z - Xv.z) * xq_inv;
-        const float gx = q.x - uh[s][0];
-        const float gy = q.y - uh[s][1];
-        const float gz = q.z - uh[s][2];
-        const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
-        // l^2 - l0^2 = (d0 + d) . delta
-        const float sl = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
-        const float e2 = sl * (il * il);
-        const unsigned rid = (uint32_t)wv >> 24;
-        float t;
-        if (lin_only)
-          t = lin_EA_half * il * e2;
-        else if (mix2 && j < Dn[s])
-          t = react32_mix2(nlq, ea_half_lin, (int)rid == nl_rid, e2, il);
-        else if (mix2)
-          t = ea_half_lin * il * e2;
-        else
-          t = react32_tab(rtab, rid, e2, il);
-        fx -= dx * t;
-        fy -= dy * t;
-        fz -= dz * t;
+      };
+      // the prefetch index is clamped to the slice's last entry (entries
+      // past it reload that one), so the loads need no branch
+      const int Dm = D - 1;
+      int32_t wv_n, wv_nn;
+      float il_n, il_nn;
+      entry(0, wv_n, il_n);
+      entry(min(1, Dm), wv_nn, il_nn);
+      // entries [j0, j1) under one law, chosen outside the loop: the loop
+      // body then carries no mode tests and no divergent branch
+      auto run = [&](int j0, int j1, auto LAWC) __attribute__((always_inline)) {
+        constexpr int LAW = decltype(LAWC)::value;
+        for (int j = j0; j < j1; ++j) {
+          const int32_t wv = wv_n;
+          const float il = il_n;
+          wv_n = wv_nn;
+          il_n = il_nn;
+          entry(min(j + 2, Dm), wv_nn, il_nn);
+          const int w = wv & 0xFFFFFF;
+          const float4 q = ld16(P + w);
+          const int4 xq = ld16(Xs + w);
+          // reference vector: exact integer difference, one rounding to FP32
+          const float ex = (float)(xq.x - Xv.x) * xq_inv;
+          const float ey = (float)(xq.y - Xv.y) * xq_inv;
+          const float ez = (float)(xq.z - Xv.z) * xq_inv;
+          const float gx = q.x - uh[s][0];
+          const float gy = q.y - uh[s][1];
+          const float gz = q.z - uh[s][2];
+          const float dx = ex + gx, dy = ey + gy, dz = ez + gz;
+          // l^2 - l0^2 = (d0 + d) . delta
+          const float sl = dot3f(ex + dx, ey + dy, ez + dz, gx, gy, gz);
+          const float e2 = sl * (il * il);
+          float t;
+          if (LAW == kLawLin) {
+            t = lin_EA_half * il * e2;
+          } else if (LAW == kLawMixNl) {
+            // both laws on every lane, then a select
+            const float tl = ea_half_lin * il * e2;
+            const float tn = toe_ext_t(nlq, e2, il);
+            t = (int)((uint32_t)wv >> 24) == nl_rid ? tn : tl;
+          } else if (LAW == kLawMixLin) {
+            t = ea_half_lin * il * e2;
+          } else {
+            t = react32_tab(rtab, (uint32_t)wv >> 24, e2, il);
+          }
+          fx -= dx * t;
+          fy -= dy * t;
+          fz -= dz * t;
+        }
+      };
+      if (lin_only) {
+        run(0, D, std::integral_constant<int, kLawLin>{});
+      } else if (mix2) {
+        const int dn = min(Dn[s], D);
+        run(0, dn, std::integral_constant<int, kLawMixNl>{});
+        run(dn, D, std::integral_constant<int, kLawMixLin>{});
+      } else {
+        run(0, D, std::integral_constant<int, kLawTab>{});
       }
     }
     f[0] = fx;
@@ -1587,6 +1631,10 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   const float *gil = a.ell_il + R.ell_off;
   const double *gl0 = a.ell_l0 + R.ell_off;
   const int32_t *gfr = a.ell_fref + R.ell_off;
+  // the free nodes' force-record references as a raw buffer (loads past the
+  // end return 0; every load here stays within the slices)
+  const __amdgpu_buffer_rsrc_t rsfr = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<int32_t *>(gfr), 0, R.ell_free_entries * (int)sizeof(int32_t), kBufRsrcWord3);
   const double *X = a.X + 3 * nb;
   if (tid < R.n_react) pack_react_table(a.reacts[R.react_off + tid], rtab + kReactTabStride * tid);
   const double EA_half = 0.5 * a.reacts[R.react_off].E * a.reacts[R.react_off].A;
@@ -1781,13 +1829,25 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
         default:
           break;
       }
-      for (int j = kLaneDeg; j < D; ++j) {
-        const int32_t fr = E[j * kWave];
-        const F3 v = ffr[fr & 0x7FFFFFFF];
-        const float sg = fr < 0 ? -1.0f : 1.0f;
-        fx = fmaf(sg, v.x, fx);
-        fy = fmaf(sg, v.y, fy);
-        fz = fmaf(sg, v.z, fz);
+      // four references per group load together (indices clamped to the
+      // slice's last entry) and are then summed in list order
+      const int vo = (sbn[s] + lane) * (int)sizeof(int32_t);
+      for (int j = kLaneDeg; j < D; j += 4) {
+        int32_t fr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          fr[q] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(
+              rsfr, vo, min(j + q, D - 1) * (kWave * (int)sizeof(int32_t)), 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (j + q < D) {
+            const F3 v = ffr[fr[q] & 0x7FFFFFFF];
+            const float sg = fr[q] < 0 ? -1.0f : 1.0f;
+            fx = fmaf(sg, v.x, fx);
+            fy = fmaf(sg, v.y, fy);
+            fz = fmaf(sg, v.z, fz);
+          }
+        }
       }
     } else if (D > 0) {
       int32_t nx = E[0];
