@@ -214,6 +214,15 @@ __device__ inline float ext_law(float K, float C, float il, float ilam) {
 constexpr float kToePolyMax = 0.25f;  // |x| bound of the degree-7 expm1
 constexpr int kReactTabStride = 3;    // float4 per reaction
 
+// Law of a run of list entries in the kernels that stream incidence lists:
+// the one linear reaction, the nonlinear-first entries of a
+// one-linear-one-nonlinear RVE (either law per lane), its later entries
+// (linear on every lane), or the packed table.
+enum { kLawLin = 0, kLawMixNl = 1, kLawMixLin = 2, kLawTab = 3 };
+// Fourth word of a raw buffer resource on gfx9-family GPUs: 32-bit data
+// format (a zero format makes untyped buffer loads return 0).
+constexpr int kBufRsrcWord3 = 0x00020000;
+
 __device__ inline void pack_react_table(const ReactDev &q, float4 *t) {
   const bool nl = q.type == MF_REACTION_NONLINEAR;
   t[0] = make_float4(q.EA_half, nl ? q.alpha : 0.0f, nl ? q.B_half : 0.0f, nl ? 1.0f : 0.0f);

@@ -11,14 +11,6 @@ namespace kern {
 // RM_LIN: the RVE has one linear reaction; RM_MIX2: at most one linear and
 // one nonlinear reaction, toe range within the expm1 polynomial's bound.
 enum { RM_TABLE = 0, RM_LIN = 1, RM_MIX2 = 2 };
-// Law of a run of list entries in the LDS-list kernel: the one linear
-// reaction, the nonlinear-first entries of a one-linear-one-nonlinear RVE
-// (either law per lane), its later entries (linear on every lane), or the
-// packed table.
-enum { kLawLin = 0, kLawMixNl = 1, kLawMixLin = 2, kLawTab = 3 };
-// Fourth word of a raw buffer resource on gfx9-family GPUs: 32-bit data
-// format (a zero format makes untyped buffer loads return 0).
-constexpr int kBufRsrcWord3 = 0x00020000;
 
 // Pointers into LDS (address space 3): byte addresses within the
 // workgroup's LDS allocation.
