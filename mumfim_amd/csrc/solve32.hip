@@ -1781,9 +1781,9 @@ __global__ __launch_bounds__(B) void solve32s(SolveArgs a) {
   // +force where it is end b, summed in list order.  RR: the first kLaneDeg
   // references come from registers and the wave-uniform slice width selects
   // a straight-line body, so the record loads issue together; entries past
-  // kLaneDeg (high-degree slices only) stream their references from global
-  // memory.  Otherwise the references stream from global memory one entry
-  // ahead.
+  // kLaneDeg (high-degree slices only) load their references from global
+  // memory four at a time.  Otherwise the references stream from global
+  // memory one entry ahead.
   auto node_force = [&](int s, float (&f)[3]) {
     const int D = Dn[s];
     const int32_t *E = gfr + sbn[s] + lane;
